@@ -1,0 +1,196 @@
+/*
+ * ls_api.h -- C ABI of the MI355X (gfx950) G-equation level-set solver.
+ *
+ *   dG/dt + u . grad G = S_L |grad G|
+ *
+ * advanced with upwinded WENO-5 on the effective velocity
+ * u_eff = u - S_L grad_c G / sqrt(|grad_c G|^2 + eps), TVD-RK3 (Shu-Osher) and
+ * HCR-2 constrained reinitialisation, on a padded FP64 grid (3 ghost layers
+ * per side, x fastest).
+ *
+ * Conventions
+ *   - Every entry point returns LS_OK (0) or a negative LS_ERR_* code; nothing
+ *     calls exit().  ls_last_error() returns a message for the last failure on
+ *     the calling thread.
+ *   - All field pointers are DEVICE pointers to caller-owned padded arrays of
+ *     nx_t*ny_t[*nz_t] doubles, nx_t = nx + 2*nghost, index (k*ny_t + j)*nx_t + i.
+ *     The caller allocates them; the solver never frees them.  Solver-owned
+ *     workspace is allocated by ls_create and freed by ls_destroy.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Work is stream-ordered; entry points return without synchronising
+ *     unless their comment says otherwise.
+ *   - One solver handle per host thread at a time; calls that share a handle
+ *     from two threads race.
+ *
+ * Reference seams replaced (juhoon196/Level-set-method-code):
+ *   SimParams / initDefaultParams / calculateTimeStep
+ *       Local_code/level-set_MPI_2D/include/config.h:123-263
+ *       Local_code/level-set_MPI_3D/include/config.h:134-293
+ *   rk3TimeStep(G_n, G_new, G_1, G_2, G_rhs, u, v, params)
+ *       Local_code/level-set_MPI_2D/include/rk3.h:168-187
+ *   rk3TimeStep3D(..., u, v, w, params)
+ *       Local_code/level-set_MPI_3D/include/rk3.h:131-150
+ *   applyBoundaryConditions / applyBoundaryConditions3D
+ *       Local_code/level-set_MPI_2D/include/boundary.h:217-229
+ *       Local_code/level-set_MPI_3D/include/boundary.h:250-261
+ *   reinitializeWithSwap / reinitializeWithSwap3D
+ *       Local_code/level-set_MPI_2D/include/reinitialization.h:321-360
+ *       Local_code/level-set_MPI_3D/include/reinitialization.h:199-228
+ *   time loop of runStandaloneSimulation[3D]
+ *       Local_code/level-set_MPI_2D/src/main.cpp:310-354
+ *       Local_code/level-set_MPI_3D/src/main.cpp:287-334
+ *   computeL2Error / computeInterfaceArea / computeInterfaceVolume
+ *       Local_code/level-set_MPI_2D/include/io.h:194-246
+ *       Local_code/level-set_MPI_3D/include/io.h:229-288
+ *   saveFieldBinary / saveFieldBinary3D
+ *       Local_code/level-set_MPI_2D/include/io.h:34-114
+ *       Local_code/level-set_MPI_3D/include/io.h:34-136
+ */
+#ifndef LS_API_H
+#define LS_API_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LS_OK 0
+#define LS_ERR_INVALID (-1)     /* bad parameter or shape */
+#define LS_ERR_NOMEM (-2)       /* device allocation failed */
+#define LS_ERR_DEVICE (-3)      /* HIP runtime error */
+#define LS_ERR_IO (-4)          /* file could not be written */
+#define LS_ERR_NONFINITE (-5)   /* a NaN or Inf was found by ls_check_finite */
+
+/* Boundary kinds per axis (both sides of an axis share the kind). */
+#define LS_BC_PERIODIC 0        /* boundary.h:98-114: ghost g <- interior n+g */
+#define LS_BC_ZERO_GRAD 1       /* boundary.h:126-168: copy of the edge cell */
+#define LS_BC_LIN_EXTRAP 2      /* boundary.h:177-200: y axis only */
+
+/* Arithmetic modes. */
+#define LS_MODE_EXACT 0         /* the reference's operation order, no FMA,
+                                   correctly rounded divisions: bitwise equal
+                                   to the contraction-free CPU build */
+#define LS_MODE_FAST 1          /* same scheme, reorganised arithmetic
+                                   (shared smoothness indicators, reciprocal
+                                   multiplies, FMA); within 1e-12 relative */
+
+/* Velocity handling inside ls_advance. */
+#define LS_VEL_ARRAYS 0         /* caller arrays, constant in time */
+#define LS_VEL_DEFORM3D 1       /* recompute the 3D deformation field at the
+                                   start of every step, period vel_period
+                                   (3D main.cpp:296-299, initial_conditions.h:232-280) */
+
+typedef struct ls_params {
+    int32_t dim;                /* 2 or 3 */
+    int32_t nx, ny, nz;         /* interior points; nz ignored in 2D */
+    int32_t nghost;             /* must be 3 (WENO-5 stencil) */
+    int32_t bc[3];              /* LS_BC_* for x, y, z */
+    int32_t mode;               /* LS_MODE_* */
+    int32_t velocity;           /* LS_VEL_* */
+    double dx, dy, dz;          /* spacing; reference: L/(N-1) */
+    double x0, y0, z0;          /* coordinate of the first interior point */
+    double s_l;                 /* laminar flame speed (scalar) */
+    double eps;                 /* 1e-10: S_L gate and |grad| regulariser */
+    int32_t reinit_interval;    /* HCR-2 after steps ri+1, 2ri+1, ...; 0 = off */
+    int32_t reinit_iters;       /* pseudo-time iterations (reference default 2) */
+    double reinit_dtau;         /* 0.25*min(dx,dy[,dz]) in the reference */
+    double reinit_beta;         /* 0.5 */
+    double vel_period;          /* T for LS_VEL_DEFORM3D */
+} ls_params_t;
+
+typedef struct ls_solver ls_solver_t;
+
+/* Fill `p` with the reference defaults for a dim-D grid of n^dim interior
+ * points on the unit box: dx = 1/(n-1), eps 1e-10, HCR-2 dtau 0.25*dx,
+ * beta 0.5, 2 iterations, interval 10 (2D: periodic x + zero-gradient y;
+ * 3D: periodic), exact mode.  Replaces initDefaultParams (config.h:166-213). */
+int ls_default_params(ls_params_t* p, int32_t dim, int32_t n);
+
+/* dt = cfl*min(dx,dy[,dz]) / (|u|+|v|[+|w|]+S_L) with the reference's
+ * fallbacks for a vanishing speed (2D: eps, 3D: 2.0).  Replaces
+ * calculateTimeStep (2D config.h:251-263, 3D config.h:272-293). */
+double ls_cfl_dt(const ls_params_t* p, double cfl, double u, double v, double w);
+
+/* Create a solver on the current HIP device; allocates two padded fields of
+ * workspace plus reduction scratch. */
+int ls_create(const ls_params_t* p, ls_solver_t** out);
+int ls_destroy(ls_solver_t* s);
+/* Number of doubles in one padded field. */
+int64_t ls_field_size(const ls_params_t* p);
+
+/* Fill every ghost cell of G from its interior (applyBoundaryConditions). */
+int ls_fill_ghosts(ls_solver_t* s, double* G, void* stream);
+
+/* One TVD-RK3 step G_new = RK3(G_n) with time step dt, ghosts filled after
+ * every stage (rk3TimeStep / rk3TimeStep3D).  w is ignored in 2D.  sl_field
+ * NULL uses the scalar s_l; otherwise a padded per-cell flame speed gated per
+ * cell by s_l > eps.  G_new must not alias G_n. */
+int ls_rk3_step(ls_solver_t* s, const double* G_n, double* G_new, const double* u,
+                const double* v, const double* w, const double* sl_field, double dt,
+                void* stream);
+
+/* HCR-2 reinitialisation of G in place: reinit_iters pseudo-time steps with
+ * phi_0 = G and a ghost fill after each (reinitializeWithSwap). */
+int ls_reinit_hcr2(ls_solver_t* s, double* G, void* stream);
+
+/* The reference time loop on G in place:
+ *   while (t < t_final && step < step0 + max_steps):
+ *     if t + dt > t_final: dt = t_final - t
+ *     [LS_VEL_DEFORM3D: u,v,w <- deformation field at time t]
+ *     G <- RK3(G);  if ri > 0 && step > 0 && step % ri == 0: HCR-2(G)
+ *     t += dt; step++
+ * step counts from step0 (the global step index of the first step).  On
+ * return *t_io holds the final time, *steps_done the number of steps and
+ * *dt_last the last dt used (any of them may be NULL except t_io).
+ * Synchronises the stream before returning. */
+int ls_advance(ls_solver_t* s, double* G, double* u, double* v, double* w,
+               const double* sl_field, double dt, double t_final, int64_t step0,
+               int64_t max_steps, double* t_io, int64_t* steps_done, double* dt_last,
+               void* stream);
+
+/* Diagnostics over interior cells: *l2 = sqrt(mean((G-G0)^2)) (G0 may be
+ * NULL, then l2 is not computed) and *area = count(G < 0) * dx*dy[*dz].
+ * Deterministic summation order.  Synchronises the stream. */
+int ls_diag(ls_solver_t* s, const double* G, const double* G0, double* l2, double* area,
+            void* stream);
+
+/* LS_OK if every interior value of G is finite, LS_ERR_NONFINITE otherwise.
+ * Synchronises the stream. */
+int ls_check_finite(ls_solver_t* s, const double* G, void* stream);
+
+/* Write a HOST copy of a padded field in the reference's binary G format:
+ * int32 {nx, ny, nghost} (2D) or {nx, ny, nz, nghost} (3D), then the padded
+ * doubles with the y-ghost rows replaced by the adjacent interior row. */
+int ls_write_G(const ls_params_t* p, const double* host_G, const char* path);
+
+/* Synthetic inputs generated on the device (padded arrays, ghosts included).
+ *   LS_CASE_CIRCLE2D        circle r=0.25 at (0.5,0.5), u=(0.5,0)
+ *   LS_CASE_SPHERE_SWIRL3D  sphere r=0.2 at (0.5,0.5,0.5),
+ *                           u=(sin pi x cos pi y, -cos pi x sin pi y, 0.2 sin 2 pi z)
+ *   LS_CASE_PLANE_MODES3D   G = x-0.3-eta(y,z), eta = 8 seeded modes a~U[0,0.02],
+ *                           k in {1..8}^2; u = (1,0,0) + 16 seeded solenoidal
+ *                           modes of amplitude 0.1, integer |k| in [1,4]
+ *   LS_CASE_ZALESAK2D       reference Zalesak disk (0.5,0.75) r 0.15, slot
+ *                           0.05 x 0.25, rotation omega = 2 pi / period
+ *   LS_CASE_SPHERE_DEFORM3D reference sphere (0.35,0.35,0.35) r 0.15 with the
+ *                           deformation velocity at t = 0, period `period`
+ *   LS_CASE_FLAME2D         seeded circle or plane G0, u = uniform + 8 seeded
+ *                           modes of amplitude 0.1 (one ensemble member)
+ * w may be NULL in 2D.  `seed` selects the seeded draws (SplitMix64). */
+#define LS_CASE_CIRCLE2D 1
+#define LS_CASE_SPHERE_SWIRL3D 2
+#define LS_CASE_PLANE_MODES3D 3
+#define LS_CASE_ZALESAK2D 4
+#define LS_CASE_SPHERE_DEFORM3D 5
+#define LS_CASE_FLAME2D 6
+int ls_init_case(const ls_params_t* p, int32_t case_id, uint64_t seed, double period,
+                 double* G, double* u, double* v, double* w, void* stream);
+
+/* Message for the last error on this thread (never NULL). */
+const char* ls_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LS_API_H */

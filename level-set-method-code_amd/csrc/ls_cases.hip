@@ -1,0 +1,288 @@
+// ls_cases.hip -- synthetic inputs generated on the device.
+//
+// Shapes and velocity fields of the reference test cases
+// (Local_code/level-set_MPI_2D/include/initial_conditions.h:67-117,209-281;
+//  Local_code/level-set_MPI_3D/include/initial_conditions.h:52-58,232-343)
+// and the seeded workloads of the benchmark configurations.  Seeded draws use
+// SplitMix64 on the host; the kernels receive the drawn mode tables by value.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "ls_api.h"
+#include "ls_internal.hpp"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct SplitMix64 {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+constexpr int kMaxModes = 16;
+
+// Mode tables: G-surface modes (amplitude, wavenumbers, phase) and solenoidal
+// velocity modes (unit amplitude direction a_hat perpendicular to k).
+struct Modes {
+    int ng_modes, nv_modes;
+    double ga[kMaxModes], gky[kMaxModes], gkz[kMaxModes], gph[kMaxModes];
+    double vk[kMaxModes][3], va[kMaxModes][3], vph[kMaxModes];
+    double vamp;
+    double base[3];
+    // flame2d shape
+    int circle;
+    double cx, cy, r, x_off;
+};
+
+struct Coord {
+    double x0, y0, z0, dx, dy, dz;
+    int ng, nxt, nyt, nzt;
+};
+
+__device__ __forceinline__ void coord(const Coord& c, int i, int j, int k, double& x, double& y, double& z) {
+    x = c.x0 + (i - c.ng) * c.dx;
+    y = c.y0 + (j - c.ng) * c.dy;
+    z = c.z0 + (k - c.ng) * c.dz;
+}
+
+__device__ __forceinline__ void deformation(double x, double y, double z, double t, double T,
+                                            double& u, double& v, double& w) {
+    // initial_conditions.h:232-247 (3D)
+    const double tf = cos(kPi * t / T);
+    const double spx = sin(kPi * x), s2px = sin(2.0 * kPi * x);
+    const double spy = sin(kPi * y), s2py = sin(2.0 * kPi * y);
+    const double spz = sin(kPi * z), s2pz = sin(2.0 * kPi * z);
+    u = 2.0 * spx * spx * s2py * s2pz * tf;
+    v = -s2px * spy * spy * s2pz * tf;
+    w = -s2px * s2py * spz * spz * tf;
+}
+
+__device__ __forceinline__ double zalesak2d(double x, double y, double cx, double cy, double r,
+                                            double sw, double sl) {
+    // initial_conditions.h:91-117 (2D): disk minus a slot open at the bottom.
+    const double ddx = x - cx, ddy = y - cy;
+    const double disk = sqrt(ddx * ddx + ddy * ddy) - r;
+    const double hw = sw / 2.0;
+    const double bot = cy - r, top = bot + sl;
+    const double sx = fabs(x - cx) - hw;
+    const double sy = fmax(bot - y, y - top);
+    double slot;
+    if (sx < 0.0 && sy < 0.0) slot = fmax(sx, sy);
+    else if (sx < 0.0) slot = sy;
+    else if (sy < 0.0) slot = sx;
+    else slot = sqrt(sx * sx + sy * sy);
+    return fmax(disk, -slot);
+}
+
+__global__ __launch_bounds__(256) void k_case(Coord c, int case_id, Modes m, double period,
+                                              double* __restrict__ G, double* __restrict__ u,
+                                              double* __restrict__ v, double* __restrict__ w,
+                                              int64_t ntot) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot) return;
+    const int i = (int)(t % c.nxt);
+    const int j = (int)((t / c.nxt) % c.nyt);
+    const int k = (int)(t / ((int64_t)c.nxt * c.nyt));
+    double x, y, z;
+    coord(c, i, j, k, x, y, z);
+    double g = 0.0, uu = 0.0, vv = 0.0, ww = 0.0;
+    switch (case_id) {
+        case LS_CASE_CIRCLE2D: {
+            const double a = x - 0.5, b = y - 0.5;
+            g = sqrt(a * a + b * b) - 0.25;
+            uu = 0.5;
+            break;
+        }
+        case LS_CASE_SPHERE_SWIRL3D: {
+            const double a = x - 0.5, b = y - 0.5, d = z - 0.5;
+            g = sqrt(a * a + b * b + d * d) - 0.2;
+            uu = sin(kPi * x) * cos(kPi * y);
+            vv = -cos(kPi * x) * sin(kPi * y);
+            ww = 0.2 * sin(2.0 * kPi * z);
+            break;
+        }
+        case LS_CASE_PLANE_MODES3D: {
+            double eta = 0.0;
+            for (int q = 0; q < m.ng_modes; q++)
+                eta += m.ga[q] * sin(2.0 * kPi * (m.gky[q] * y + m.gkz[q] * z) + m.gph[q]);
+            g = x - 0.3 - eta;
+            uu = m.base[0];
+            vv = m.base[1];
+            ww = m.base[2];
+            for (int q = 0; q < m.nv_modes; q++) {
+                const double s = m.vamp * sin(2.0 * kPi * (m.vk[q][0] * x + m.vk[q][1] * y + m.vk[q][2] * z) + m.vph[q]);
+                uu += s * m.va[q][0];
+                vv += s * m.va[q][1];
+                ww += s * m.va[q][2];
+            }
+            break;
+        }
+        case LS_CASE_ZALESAK2D: {
+            g = zalesak2d(x, y, 0.5, 0.75, 0.15, 0.05, 0.25);
+            const double om = 2.0 * kPi / period;
+            uu = -om * (y - 0.5);
+            vv = om * (x - 0.5);
+            break;
+        }
+        case LS_CASE_SPHERE_DEFORM3D: {
+            const double a = x - 0.35, b = y - 0.35, d = z - 0.35;
+            g = sqrt(a * a + b * b + d * d) - 0.15;
+            deformation(x, y, z, 0.0, period, uu, vv, ww);
+            break;
+        }
+        case LS_CASE_FLAME2D: {
+            if (m.circle) {
+                const double a = x - m.cx, b = y - m.cy;
+                g = sqrt(a * a + b * b) - m.r;
+            } else {
+                g = x - m.x_off;
+            }
+            uu = m.base[0];
+            vv = m.base[1];
+            for (int q = 0; q < m.nv_modes; q++) {
+                const double s = m.vamp * sin(2.0 * kPi * (m.vk[q][0] * x + m.vk[q][1] * y) + m.vph[q]);
+                uu += s * m.va[q][0];
+                vv += s * m.va[q][1];
+            }
+            break;
+        }
+        default:
+            break;
+    }
+    G[t] = g;
+    u[t] = uu;
+    v[t] = vv;
+    if (w) w[t] = ww;
+}
+
+__global__ __launch_bounds__(256) void k_deform(Coord c, double tt, double T, double* __restrict__ u,
+                                                double* __restrict__ v, double* __restrict__ w,
+                                                int64_t ntot) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot) return;
+    const int i = (int)(t % c.nxt);
+    const int j = (int)((t / c.nxt) % c.nyt);
+    const int k = (int)(t / ((int64_t)c.nxt * c.nyt));
+    double x, y, z;
+    coord(c, i, j, k, x, y, z);
+    deformation(x, y, z, tt, T, u[t], v[t], w[t]);
+}
+
+Coord make_coord(const ls_params_t& p) {
+    Coord c;
+    c.x0 = p.x0;
+    c.y0 = p.y0;
+    c.z0 = p.z0;
+    c.dx = p.dx;
+    c.dy = p.dy;
+    c.dz = p.dim == 3 ? p.dz : 0.0;
+    c.ng = p.nghost;
+    c.nxt = p.nx + 2 * p.nghost;
+    c.nyt = p.ny + 2 * p.nghost;
+    c.nzt = p.dim == 3 ? p.nz + 2 * p.nghost : 1;
+    return c;
+}
+
+// Integer wavevector with 1 <= |k| <= 4 (components in [-4,4]; 2D: kz = 0),
+// then a unit direction perpendicular to it (Gram-Schmidt on a random vector).
+void draw_solenoidal(SplitMix64& rng, int dim, double k[3], double a[3]) {
+    for (;;) {
+        k[0] = std::floor(9.0 * rng.uniform()) - 4.0;
+        k[1] = std::floor(9.0 * rng.uniform()) - 4.0;
+        k[2] = dim == 3 ? std::floor(9.0 * rng.uniform()) - 4.0 : 0.0;
+        const double n2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        if (n2 >= 1.0 && n2 <= 16.0) break;
+    }
+    const double kn = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+    const double kh[3] = {k[0] / kn, k[1] / kn, k[2] / kn};
+    for (;;) {
+        double r[3] = {rng.uniform() - 0.5, rng.uniform() - 0.5, dim == 3 ? rng.uniform() - 0.5 : 0.0};
+        const double d = r[0] * kh[0] + r[1] * kh[1] + r[2] * kh[2];
+        for (int q = 0; q < 3; q++) r[q] -= d * kh[q];
+        const double rn = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        if (rn > 1e-3) {
+            for (int q = 0; q < 3; q++) a[q] = r[q] / rn;
+            return;
+        }
+    }
+}
+
+}  // namespace
+
+namespace lsi {
+
+int launch_deformation_velocity(const ls_params_t& p, double t, double period, double* u, double* v,
+                                double* w, hipStream_t st) {
+    const Coord c = make_coord(p);
+    const int64_t n = (int64_t)c.nxt * c.nyt * c.nzt;
+    hipLaunchKernelGGL(k_deform, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, t, period, u, v, w, n);
+    if (hipGetLastError() != hipSuccess) return set_error(LS_ERR_DEVICE, "deformation velocity launch failed");
+    return LS_OK;
+}
+
+}  // namespace lsi
+
+extern "C" int ls_init_case(const ls_params_t* p, int32_t case_id, uint64_t seed, double period,
+                            double* G, double* u, double* v, double* w, void* stream) {
+    int rc = lsi::validate(p);
+    if (rc) return rc;
+    if (!G || !u || !v) return lsi::set_error(LS_ERR_INVALID, "NULL field pointer");
+    const bool is3 = case_id == LS_CASE_SPHERE_SWIRL3D || case_id == LS_CASE_PLANE_MODES3D ||
+                     case_id == LS_CASE_SPHERE_DEFORM3D;
+    const bool is2 = case_id == LS_CASE_CIRCLE2D || case_id == LS_CASE_ZALESAK2D || case_id == LS_CASE_FLAME2D;
+    if (!is3 && !is2) return lsi::set_error(LS_ERR_INVALID, "unknown case id");
+    if ((is3 && p->dim != 3) || (is2 && p->dim != 2))
+        return lsi::set_error(LS_ERR_INVALID, "case dimension does not match params");
+    if (is3 && !w) return lsi::set_error(LS_ERR_INVALID, "w is NULL for a 3D case");
+    if ((case_id == LS_CASE_ZALESAK2D || case_id == LS_CASE_SPHERE_DEFORM3D) && !(period > 0.0))
+        return lsi::set_error(LS_ERR_INVALID, "period must be > 0");
+
+    Modes m;
+    std::memset(&m, 0, sizeof(m));
+    SplitMix64 rng{seed};
+    if (case_id == LS_CASE_PLANE_MODES3D) {
+        m.ng_modes = 8;
+        for (int q = 0; q < 8; q++) {
+            m.ga[q] = 0.02 * rng.uniform();
+            m.gky[q] = 1.0 + std::floor(8.0 * rng.uniform());
+            m.gkz[q] = 1.0 + std::floor(8.0 * rng.uniform());
+            m.gph[q] = 2.0 * kPi * rng.uniform();
+        }
+        m.nv_modes = 16;
+        m.vamp = 0.1;
+        m.base[0] = 1.0;
+        for (int q = 0; q < 16; q++) {
+            draw_solenoidal(rng, 3, m.vk[q], m.va[q]);
+            m.vph[q] = 2.0 * kPi * rng.uniform();
+        }
+    } else if (case_id == LS_CASE_FLAME2D) {
+        m.circle = rng.uniform() < 0.5 ? 1 : 0;
+        m.cx = 0.3 + 0.4 * rng.uniform();
+        m.cy = 0.3 + 0.4 * rng.uniform();
+        m.r = 0.1 + 0.1 * rng.uniform();
+        m.x_off = 0.2 + 0.2 * rng.uniform();
+        m.nv_modes = 8;
+        m.vamp = 0.1;
+        m.base[0] = 0.5;
+        for (int q = 0; q < 8; q++) {
+            draw_solenoidal(rng, 2, m.vk[q], m.va[q]);
+            m.vph[q] = 2.0 * kPi * rng.uniform();
+        }
+    }
+    const Coord c = make_coord(*p);
+    const int64_t n = (int64_t)c.nxt * c.nyt * c.nzt;
+    hipLaunchKernelGGL(k_case, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), c, (int)case_id, m, period, G, u, v,
+                       p->dim == 3 ? w : nullptr, n);
+    if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "case launch failed");
+    return LS_OK;
+}

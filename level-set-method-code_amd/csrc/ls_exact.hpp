@@ -1,0 +1,185 @@
+// ls_exact.hpp -- device numerics of the exact mode.
+//
+// Every expression keeps the reference's C++ evaluation order and is compiled
+// with -ffp-contract=off, so that each cell reproduces the contraction-free
+// CPU result bit for bit (IEEE double, correctly rounded division and sqrt).
+//
+//   smoothness indicators  Local_code/level-set_MPI_3D/include/weno5.h:28-37
+//   weno5_left / right     Local_code/level-set_MPI_3D/include/weno5.h:39-81
+//   upwind axis derivative Local_code/level-set_MPI_3D/include/weno5.h:90-196
+//   u_eff RHS              Local_code/level-set_MPI_3D/include/rk3.h:31-81
+//                          Local_code/level-set_MPI_2D/include/rk3.h:54-98
+//   Godunov |grad phi|     Local_code/level-set_MPI_3D/include/weno5.h:202-248
+//   HCR-2 step             Local_code/level-set_MPI_3D/include/reinitialization.h:46-193
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lsx {
+
+constexpr double kWenoEps = 1.0e-6;
+
+// Kernel-side geometry, passed by value.
+struct Geom {
+    int32_t dim, nx, ny, nz, ng;
+    int32_t nxt, nyt, nzt;
+    int32_t bc[3];
+    int64_t sy, sz;      // strides of y and z in doubles
+    double dx, dy, dz;
+    double s_l, eps;
+    double dtau, beta;
+};
+
+// beta_k of the three 3-point sub-stencils of (a,b,c,d,e).
+__device__ __forceinline__ double beta3(double p, double q, double r) {
+    double d = r - 2.0 * q + p;
+    return (13.0 / 12.0) * d * d + 0.25 * (r - p) * (r - p);
+}
+
+// weno5_left on (v0..v4): reconstruction at the right face of v2.
+__device__ __forceinline__ double weno_l(double v0, double v1, double v2, double v3, double v4) {
+    double b0 = beta3(v0, v1, v2);
+    double b1 = beta3(v1, v2, v3);
+    double b2 = beta3(v2, v3, v4);
+    double a0 = 0.1 / ((kWenoEps + b0) * (kWenoEps + b0));
+    double a1 = 0.6 / ((kWenoEps + b1) * (kWenoEps + b1));
+    double a2 = 0.3 / ((kWenoEps + b2) * (kWenoEps + b2));
+    double s = a0 + a1 + a2;
+    double w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+    double p0 = (2.0 * v0 - 7.0 * v1 + 11.0 * v2) / 6.0;
+    double p1 = (-v1 + 5.0 * v2 + 2.0 * v3) / 6.0;
+    double p2 = (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
+// weno5_right on (v0..v4): the indicators are taken on the mirrored stencil
+// (v4,v3,v2,v1,v0), so beta_0 uses (v4,v3,v2), beta_1 (v3,v2,v1), beta_2 (v2,v1,v0).
+__device__ __forceinline__ double weno_r(double v0, double v1, double v2, double v3, double v4) {
+    double b0 = beta3(v4, v3, v2);
+    double b1 = beta3(v3, v2, v1);
+    double b2 = beta3(v2, v1, v0);
+    double a0 = 0.3 / ((kWenoEps + b0) * (kWenoEps + b0));
+    double a1 = 0.6 / ((kWenoEps + b1) * (kWenoEps + b1));
+    double a2 = 0.1 / ((kWenoEps + b2) * (kWenoEps + b2));
+    double s = a0 + a1 + a2;
+    double w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+    double p0 = (11.0 * v2 - 7.0 * v3 + 2.0 * v4) / 6.0;
+    double p1 = (2.0 * v1 + 5.0 * v2 - v3) / 6.0;
+    double p2 = (-v0 + 5.0 * v1 + 2.0 * v2) / 6.0;
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
+// Upwinded WENO-5 derivative from the 7-point axis stencil m3..p3.
+__device__ __forceinline__ double dweno(double m3, double m2, double m1, double c0, double p1,
+                                        double p2, double p3, double vel, double h) {
+    if (vel >= 0.0) {
+        double gl = weno_l(m3, m2, m1, c0, p1);
+        double gr = weno_l(m2, m1, c0, p1, p2);
+        return (gr - gl) / h;
+    }
+    double gl = weno_r(m2, m1, c0, p1, p2);
+    double gr = weno_r(m1, c0, p1, p2, p3);
+    return (gr - gl) / h;
+}
+
+// RHS -(u_eff . grad_w G) at the cell P points to.
+template <int DIM>
+__device__ __forceinline__ double rhs_cell(const Geom& g, const double* __restrict__ P, double uc,
+                                           double vc, double wc, double s_l) {
+    const int64_t sy = g.sy, sz = g.sz;
+    double gx = 0.0, gy = 0.0, gz = 0.0, gm = 1.0;
+    if (s_l > g.eps) {
+        gx = (P[1] - P[-1]) / (2.0 * g.dx);
+        gy = (P[sy] - P[-sy]) / (2.0 * g.dy);
+        if (DIM == 3) {
+            gz = (P[sz] - P[-sz]) / (2.0 * g.dz);
+            gm = sqrt(gx * gx + gy * gy + gz * gz + g.eps);
+        } else {
+            gm = sqrt(gx * gx + gy * gy + g.eps);
+        }
+    }
+    double ue = uc - s_l * gx / gm;
+    double ve = vc - s_l * gy / gm;
+    double dgx = dweno(P[-3], P[-2], P[-1], P[0], P[1], P[2], P[3], ue, g.dx);
+    double dgy = dweno(P[-3 * sy], P[-2 * sy], P[-sy], P[0], P[sy], P[2 * sy], P[3 * sy], ve, g.dy);
+    if (DIM == 3) {
+        double we = wc - s_l * gz / gm;
+        double dgz = dweno(P[-3 * sz], P[-2 * sz], P[-sz], P[0], P[sz], P[2 * sz], P[3 * sz], we, g.dz);
+        return -(ue * dgx + ve * dgy + we * dgz);
+    }
+    return -(ue * dgx + ve * dgy);
+}
+
+// Shu-Osher stage combine (rk3.h:103-150).
+template <int STAGE>
+__device__ __forceinline__ double stage_combine(double gn, double gs, double r, double dt) {
+    if (STAGE == 1) return gn + dt * r;
+    if (STAGE == 2) return 0.75 * gn + 0.25 * gs + 0.25 * dt * r;
+    return (1.0 / 3.0) * gn + (2.0 / 3.0) * gs + (2.0 / 3.0) * dt * r;
+}
+
+// One HCR-2 pseudo-time step at a cell.  Crossing flags and r~ are recomputed
+// from phi_0 (identical arithmetic to computeInterfaceCrossings, so identical
+// values to the stored arrays of the reference).
+template <int DIM>
+__device__ __forceinline__ double hcr2_cell(const Geom& g, const double* __restrict__ pn,
+                                            const double* __restrict__ p0) {
+    const int64_t off[6] = {-1, 1, -g.sy, g.sy, -g.sz, g.sz};
+    constexpr int NB = DIM == 3 ? 6 : 4;
+    const double pc = p0[0];
+    double sum0 = 0.0;
+    int cnt = 0, flag = 0;
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        double pq = p0[off[q]];
+        if (pc * pq < 0.0) {
+            sum0 += pq;
+            cnt++;
+            flag |= 1 << q;
+        }
+    }
+    double rt = 0.0;
+    if (cnt > 0 && fabs(sum0) > 1e-15) rt = pc / sum0;
+
+    const double phi = pn[0];
+    const double S = pc / sqrt(pc * pc + g.dx * g.dx);
+    double xm = (pn[0] - pn[-1]) / g.dx, xp = (pn[1] - pn[0]) / g.dx;
+    double ym = (pn[0] - pn[-g.sy]) / g.dy, yp = (pn[g.sy] - pn[0]) / g.dy;
+    double zm = 0.0, zp = 0.0;
+    if (DIM == 3) {
+        zm = (pn[0] - pn[-g.sz]) / g.dz;
+        zp = (pn[g.sz] - pn[0]) / g.dz;
+    }
+    double ax, bx, ay, by, az, bz;
+    if (S > 0.0) {
+        ax = fmax(xm, 0.0); bx = fmin(xp, 0.0);
+        ay = fmax(ym, 0.0); by = fmin(yp, 0.0);
+        az = fmax(zm, 0.0); bz = fmin(zp, 0.0);
+    } else {
+        ax = fmin(xm, 0.0); bx = fmax(xp, 0.0);
+        ay = fmin(ym, 0.0); by = fmax(yp, 0.0);
+        az = fmin(zm, 0.0); bz = fmax(zp, 0.0);
+    }
+    double s2;
+    if (DIM == 3) s2 = fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by) + fmax(az * az, bz * bz);
+    else s2 = fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by);
+    const double term = -S * (sqrt(s2) - 1.0);
+
+    double forcing = 0.0;
+    if (flag != 0) {
+        bool stable = (pn[0] > 0.0) == (pc > 0.0);
+#pragma unroll
+        for (int q = 0; q < NB; q++)
+            if ((flag >> q) & 1) stable = stable && ((pn[off[q]] > 0.0) == (p0[off[q]] > 0.0));
+        if (stable) {
+            double sum = 0.0;
+#pragma unroll
+            for (int q = 0; q < NB; q++)
+                if ((flag >> q) & 1) sum += pn[off[q]];
+            forcing = g.beta * (rt * sum - phi) / g.dx;
+        }
+    }
+    return phi + g.dtau * (term + forcing);
+}
+
+}  // namespace lsx

@@ -1,0 +1,14 @@
+// ls_fast.hpp -- launchers of the fast-mode kernels (ls_fast.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ls_exact.hpp"
+
+namespace lsf {
+template <int DIM>
+int launch_stage_fast(const lsx::Geom& g, int stage, const double* Gs, const double* Gn,
+                      double* out, const double* u, const double* v, const double* w,
+                      const double* sl, double dt, hipStream_t st);
+int launch_hcr2_fast(const lsx::Geom& g, const double* pn, const double* p0, double* pnew,
+                     hipStream_t st);
+}  // namespace lsf
