@@ -1,0 +1,271 @@
+"""Benchmark: 512^3 FP64 G-equation WENO5-RK3 + HCR-2 on one MI355X.
+
+Workload (the 3D roofline configuration): 512^3 interior cells on the unit
+cube, periodic, G0 = x - 0.3 - eta(y,z) with 8 seeded surface modes, velocity
+(1,0,0) + 16 seeded solenoidal modes of amplitude 0.1, S_L = 0.08, CFL 0.5,
+HCR-2 (2 iterations) after every 5th RK3 step.  A step is one RK3 step plus
+1/5 of an HCR-2 invocation; inputs are resident in HBM before timing starts.
+
+Prints ONE JSON line (rank 0).  Options:
+  --steps K --warmup W     timed / untimed steps (K a multiple of 5 keeps the
+                           HCR-2 share exact)
+  --mode exact|fast        arithmetic mode of the solver
+  --n N                    grid edge (default 512)
+  --no-cpu-baseline        skip the CPU reference timing
+  --profile-pass           launch a 1 GiB copy, a ghost fill and stage kernels
+                           only (for rocprofv3 --pmc runs)
+  --summarize-pmc F W      turn two rocprofv3 --pmc CSV directories (FETCH_SIZE,
+                           WRITE_SIZE) into profiles/stage_kernel_traffic_512.json
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s (WENO5-RK3+HCR2 512³ FP64), % HBM peak; adjoint vs FD grads/s"
+HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
+STAGE_BYTES = {1: 40, 2: 48, 3: 48}   # algorithmic bytes per interior cell per stage
+STEP_BYTES = 136 + 83 / 5.0           # RK3 + amortised HCR-2, bytes per cell-update
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_traffic_512.json")
+
+
+def load_ls():
+    import __graft_entry__ as ge
+    return ge.load_package()
+
+
+def setup(ls, n, mode):
+    import torch
+    p = ls.default_params(3, n, s_l=0.08, reinit_interval=5, reinit_iters=2,
+                          mode=ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT)
+    G, u, v, w = ls.init_case(p, ls.CASE_PLANE_MODES3D, seed=3)
+    s = ls.Solver(p)
+    s.fill_ghosts(G)
+    speed = (u.abs() + v.abs() + w.abs()).max().item()
+    dt = 0.5 * min(p.dx, p.dy, p.dz) / (speed + p.s_l)
+    torch.cuda.synchronize()
+    return p, s, G, u, v, w, dt
+
+
+def time_kernels(ls, s, G, u, v, w, dt, reps=10):
+    """Average device time of one stage-2 kernel, from HIP events on the launch stream."""
+    import torch
+    st = torch.cuda.current_stream()
+    a, b = s.empty(), s.empty()
+    s.rk3_stage(1, G, G, a, u, v, w, dt=dt)
+    torch.cuda.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(st)
+    for _ in range(reps):
+        s.rk3_stage(2, a, G, b, u, v, w, dt=dt)
+    e1.record(st)
+    for _ in range(reps):
+        s.fill_ghosts(b)
+    e2.record(st)
+    torch.cuda.synchronize()
+    t_stage_fill = e0.elapsed_time(e1) / reps * 1e-3
+    t_fill = e1.elapsed_time(e2) / reps * 1e-3
+    del a, b
+    return t_stage_fill - t_fill, t_fill
+
+
+def cpu_baseline(n):
+    """The reference's 3D solver (oracle/_ref/ref_bench3d, built from the
+    reference sources) under mpirun on this host's cores, on a bounded slab of
+    the workload; falls back to the OpenMP oracle restatement."""
+    cores = max(1, min(16, os.cpu_count() or 1))
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench3d")
+    mpirun = shutil.which("mpirun") or ("/opt/conda/bin/mpirun" if os.path.exists("/opt/conda/bin/mpirun") else None)
+    nz, steps = 32, 10
+    if os.path.exists(exe) and mpirun:
+        try:
+            r = subprocess.run([mpirun, "-np", str(cores), exe, str(n), str(n), str(nz), str(steps), "5"],
+                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+            line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
+            d = json.loads(line)
+            return {"value": d["cell_updates_per_s"], "unit": "cell-updates/s", "cores": cores,
+                    "kind": "reference",
+                    "sample": "reference level-set_MPI_3D solver (rk3TimeStep3D + reinitializeWithSwap3D, "
+                              "-O3 FMA) under mpirun -np %d, y-slab decomposition, %dx%dx%d periodic slab of "
+                              "the plane+modes workload, %d RK3 steps, HCR-2 every 5, %.2f s wall"
+                              % (cores, n, n, nz, steps, d["seconds"])}
+        except Exception as e:  # pragma: no cover - reported, then fall back
+            sys.stderr.write("reference CPU baseline failed (%s); using the oracle port\n" % e)
+    import numpy as np
+    from oracle.pyoracle import Oracle, make_grid
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    g = make_grid(3, n, n, 16, bc=(0, 0, 0), s_l=0.08)
+    sh = g.shape
+    x = (np.arange(sh[2]) - 3) * g.dx
+    G = np.ascontiguousarray(np.broadcast_to(x - 0.3, sh))
+    u = np.ones(sh)
+    v = np.zeros(sh)
+    w = np.zeros(sh)
+    t0 = time.perf_counter()
+    Oracle(fma=True).advance(g, G, u, v, w, dt=0.5 * g.dx / 1.08, t_final=1e30, max_steps=5,
+                             reinit_interval=5)
+    el = time.perf_counter() - t0
+    return {"value": n * n * 16 * 5 / el, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": "oracle restatement (OpenMP, %d threads), %dx%dx16 slab, 5 RK3 steps, %.2f s"
+                      % (cores, n, n, el)}
+
+
+def read_traffic(n, mode):
+    if n != 512 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    try:
+        d = json.load(open(TRAFFIC_FILE))
+        if d.get("mode") != mode:
+            return None
+        return d.get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def profile_pass(ls, n, mode):
+    """Kernels for the --pmc runs: a 1 GiB device copy (the calibration kernel
+    of known traffic: 1 GiB read + 1 GiB write), one ghost fill and three
+    stage-2 launches."""
+    import torch
+    p, s, G, u, v, w, dt = setup(ls, n, mode)
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 2.0, out=dst)  # elementwise kernel: reads 1 GiB, writes 1 GiB
+    a, b = s.empty(), s.empty()
+    s.rk3_stage(1, G, G, a, u, v, w, dt=dt)
+    for _ in range(3):
+        s.rk3_stage(2, a, G, b, u, v, w, dt=dt)
+    torch.cuda.synchronize()
+    print("profile pass done", flush=True)
+
+
+def _pmc_rows(d):
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=TRAFFIC_FILE):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) for the stage-2 kernel, each
+    scaled by what the same counter reads for the 1 GiB copy kernel."""
+    def per_kernel(rows, counter):
+        out = {}
+        for r in rows:
+            if r.get("Counter_Name") != counter:
+                continue
+            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return out
+    fr = per_kernel(_pmc_rows(fetch_dir), "FETCH_SIZE")
+    wr = per_kernel(_pmc_rows(write_dir), "WRITE_SIZE")
+    copy_k = [k for k in fr if "copy" in k.lower() or "elementwise" in k.lower()]
+    stage_k = [k for k in fr if "stage" in k.lower() and "<3, 2>" in k.replace("ILi3ELi2E", "<3, 2>")]
+    if not stage_k:
+        stage_k = [k for k in fr if "k_stage" in k and "3ELi2" in k]
+    if not copy_k or not stage_k:
+        raise SystemExit("kernels not found in pmc output: %s" % list(fr))
+    ck, sk = copy_k[0], stage_k[0]
+    gib = float(1 << 30)
+    f_scale = gib / (fr[ck][0] * 1024.0)
+    w_scale = gib / (wr[ck][0] * 1024.0)
+    fetch = sum(fr[sk]) / len(fr[sk]) * 1024.0 * f_scale
+    write = sum(wr[sk]) / len(wr[sk]) * 1024.0 * w_scale
+    cells = n ** 3
+    d = {"kernel": sk, "mode": mode, "grid": [n, n, n],
+         "fetch_kib_raw": sum(fr[sk]) / len(fr[sk]), "write_kib_raw": sum(wr[sk]) / len(wr[sk]),
+         "copy_fetch_kib_raw": fr[ck][0], "copy_write_kib_raw": wr[ck][0],
+         "fetch_scale": f_scale, "write_scale": w_scale,
+         "hbm_bytes_per_launch": fetch + write,
+         "hbm_bytes_per_cell": (fetch + write) / cells,
+         "algorithmic_bytes_per_cell": STAGE_BYTES[2]}
+    os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
+    with open(out_path, "w") as fh:
+        json.dump(d, fh, indent=1)
+    print(json.dumps(d))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-pass", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
+    a = ap.parse_args()
+
+    if a.summarize_pmc:
+        summarize_pmc(a.summarize_pmc[0], a.summarize_pmc[1], a.n, a.mode, a.traffic_out)
+        return
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus > 1 or world > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the solver would "
+              "shard by y-slabs with 3-layer halo exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    ls = load_ls()
+    torch.cuda.set_device(0)
+    if a.profile_pass:
+        profile_pass(ls, a.n, a.mode)
+        return
+
+    p, s, G, u, v, w, dt = setup(ls, a.n, a.mode)
+    cells = p.cells
+    # warmup (untimed); steps counted from 1 so that HCR-2 runs exactly once per 5 steps
+    s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=1, max_steps=a.warmup)
+    torch.cuda.synchronize()
+    step0 = 1 + a.warmup
+    st = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(st)
+    n_done, _, _ = s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=step0, max_steps=a.steps)
+    ev1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev = ev0.elapsed_time(ev1) * 1e-3
+    assert n_done == a.steps
+    elapsed = max(wall, dev)
+    value = cells * a.steps / elapsed
+    ok = s.check_finite(G) == 0
+
+    t_stage, t_fill = time_kernels(ls, s, G, u, v, w, dt)
+    ach = STAGE_BYTES[2] * cells / t_stage
+    traffic = read_traffic(a.n, a.mode)
+    roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": ach / HBM_PEAK, "traffic": traffic,
+            "kernel": "k_stage_%s<3,2> (fused WENO5/u_eff RHS + stage-2 combine)" % a.mode,
+            "kernel_ms": t_stage * 1e3, "ghost_fill_ms": t_fill * 1e3,
+            "algorithmic_bytes_per_cell": STAGE_BYTES[2],
+            "step_frac": value * STEP_BYTES / HBM_PEAK}
+    out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "3D plane+modes flame, %d^3 FP64, S_L=0.08, CFL 0.5, WENO5-RK3, "
+                                  "HCR-2 every 5 steps (2 iterations), periodic" % a.n,
+                      "grid": [a.n, a.n, a.n], "mode": a.mode, "dt": dt,
+                      "parallelism": "single GPU", "finite": ok},
+           "roofline": roof}
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -130,6 +130,18 @@ int ls_rk3_step(ls_solver_t* s, const double* G_n, double* G_new, const double* 
                 const double* v, const double* w, const double* sl_field, double dt,
                 void* stream);
 
+/* One fused RK3 stage: out = combine_k(G_n, G_s, dt * RHS(G_s)) on interior
+ * cells followed by a ghost fill of out, with RHS the u_eff WENO-5 operator:
+ *   k=1: G_n + dt*r      k=2: 0.75 G_n + 0.25 G_s + 0.25 dt r
+ *   k=3: (1/3) G_n + (2/3) G_s + (2/3) dt r      (G_s = G_n for k = 1).
+ * Replaces computeRHS + rk3Stage1/2/3 + applyBoundaryConditions
+ * (Local_code/level-set_MPI_2D/include/rk3.h:54-150,
+ *  Local_code/level-set_MPI_3D/include/rk3.h:31-125).  out must not alias
+ * G_s or G_n. */
+int ls_rk3_stage(ls_solver_t* s, int32_t stage, const double* G_s, const double* G_n,
+                 double* out, const double* u, const double* v, const double* w,
+                 const double* sl_field, double dt, void* stream);
+
 /* HCR-2 reinitialisation of G in place: reinit_iters pseudo-time steps with
  * phi_0 = G and a ghost fill after each (reinitializeWithSwap). */
 int ls_reinit_hcr2(ls_solver_t* s, double* G, void* stream);

@@ -88,6 +88,7 @@ def lib():
     L.ls_field_size.restype = C.c_int64
     L.ls_fill_ghosts.argtypes = [vp, vp, vp]
     L.ls_rk3_step.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_double, vp]
+    L.ls_rk3_stage.argtypes = [vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp, C.c_double, vp]
     L.ls_reinit_hcr2.argtypes = [vp, vp, vp]
     L.ls_advance.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int64, C.c_int64,
                              C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double), vp]
@@ -186,6 +187,11 @@ class Solver:
         self._chk(G_n, G_new, u, v, w, sl)
         check(lib().ls_rk3_step(self.h, _ptr(G_n), _ptr(G_new), _ptr(u), _ptr(v), _ptr(w),
                                 _ptr(sl), dt, _stream(stream)))
+
+    def rk3_stage(self, k, G_s, G_n, out, u, v, w=None, dt=1e-3, sl=None, stream=None):
+        self._chk(G_s, G_n, out, u, v, w, sl)
+        check(lib().ls_rk3_stage(self.h, k, _ptr(G_s), _ptr(G_n), _ptr(out), _ptr(u), _ptr(v),
+                                 _ptr(w), _ptr(sl), dt, _stream(stream)))
 
     def reinit(self, G, stream=None):
         self._chk(G)

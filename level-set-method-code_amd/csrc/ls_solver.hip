@@ -255,6 +255,8 @@ int validate(const ls_params_t* p) {
     if (p->nghost != 3) return fail(LS_ERR_INVALID, "nghost must be 3 (WENO-5 stencil)");
     const int nd = p->dim;
     const int n[3] = {p->nx, p->ny, nd == 3 ? p->nz : 3};
+    for (int a = 0; a < 3; a++)
+        if (p->bc[a] < 0 || p->bc[a] > 2) return fail(LS_ERR_INVALID, "unknown boundary kind");
     for (int a = 0; a < nd; a++) {
         if (n[a] < 3) return fail(LS_ERR_INVALID, "every interior extent must be >= 3");
         if (p->bc[a] < 0 || p->bc[a] > 2) return fail(LS_ERR_INVALID, "unknown boundary kind");
@@ -441,6 +443,17 @@ int ls_rk3_step(ls_solver_t* s, const double* G_n, double* G_new, const double* 
     if (G_new == G_n) return fail(LS_ERR_INVALID, "G_new must not alias G_n");
     if (!std::isfinite(dt)) return fail(LS_ERR_INVALID, "dt must be finite");
     return rk3(s, G_n, G_new, s->buf[0], s->buf[1], u, v, w, sl_field, dt, S(stream));
+}
+
+int ls_rk3_stage(ls_solver_t* s, int32_t k, const double* G_s, const double* G_n, double* out,
+                 const double* u, const double* v, const double* w, const double* sl_field,
+                 double dt, void* stream) {
+    if (!s || !G_s || !G_n || !out || !u || !v || (s->g.dim == 3 && !w))
+        return fail(LS_ERR_INVALID, "NULL field pointer");
+    if (k < 1 || k > 3) return fail(LS_ERR_INVALID, "stage must be 1, 2 or 3");
+    if (out == G_s || out == G_n) return fail(LS_ERR_INVALID, "out must not alias an input");
+    if (!std::isfinite(dt)) return fail(LS_ERR_INVALID, "dt must be finite");
+    return stage(s, k, G_s, G_n, out, u, v, w, sl_field, dt, S(stream));
 }
 
 int ls_reinit_hcr2(ls_solver_t* s, double* G, void* stream) {

@@ -1,0 +1,168 @@
+"""Workload-level GPU checks.
+
+1. The reference's published known answers (its run logs), reproduced by the
+   GPU exact mode:
+   - 2D Zalesak slotted disk, one rotation, N = 201..1601:
+       Local_code/level-set_MPI_2D/log/{201x201,401x401,801x801,simulation}.log:20-34
+       (identical in Local_code/level-set_GPU_2D/log/*.log)
+   - 3D sphere deformation 64^3, T = 1.5, dt = 1e-3:
+       Local_code/level-set_MPI_3D/log/simulation.log:9-34
+   The initial fields come from the oracle's FMA-contracted build, the flags
+   the logged runs were compiled with (-O3 -march=native): about 120 cells of
+   the Zalesak disk sit within 1e-15 of zero, and only the contracted
+   arithmetic reproduces the logged initial areas.
+2. Configuration 1 (2D 256^2 circle, 200 steps, HCR-2 every 10) at full size
+   and configurations 2 and 3 at reduced size: bitwise against the oracle.
+3. Configurations 2 (256^3) and 3 (512^3) at full size, where the CPU oracle
+   would take hours: periodic translation equivariance (bitwise) and
+   run-to-run determinism.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle, make_grid
+
+pytestmark = pytest.mark.gpu
+
+ZALESAK = {  # N: (steps, L2, area0, area_final) from the MPI (oracle) logs
+    201: (6284, 8.576633e-02, 0.057850, 0.058800),
+    401: (12567, 8.373019e-02, 0.058013, 0.058081),
+    801: (25133, 8.250949e-02, 0.058167, 0.058311),
+    1601: (50266, 8.243448e-02, 0.058192, 0.058205),
+}
+
+
+def _dev(cuda, a):
+    return cuda.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("N", [201, 401, 801, 1601])
+def test_zalesak_log_known_answers(ls, cuda, N):
+    steps_ref, l2_ref, a0_ref, a1_ref = ZALESAK[N]
+    T = 6.283185
+    g = make_grid(2, N, N, bc=(0, 1, 0))
+    G0, u, v = Oracle(fma=True).ic_zalesak2d(g, T)
+    p = ls.default_params(2, N, reinit_interval=0)
+    dt = ls.cfl_dt(p, 0.2, 1.0, 0.0, 0.0)  # calculateTimeStep with u_const = 1
+    assert dt == 0.2 * (1.0 / (N - 1))
+    s = ls.Solver(p)
+    dG, dG0 = _dev(cuda, G0), _dev(cuda, G0)
+    s.fill_ghosts(dG)
+    s.fill_ghosts(dG0)
+    _, area0 = s.diag(dG0)
+    steps, t, _ = s.advance(dG, _dev(cuda, u), _dev(cuda, v), dt=dt, t_final=T)
+    l2, area1 = s.diag(dG, dG0)
+    assert steps == steps_ref
+    assert "%.6e" % l2 == "%.6e" % l2_ref
+    # 401^2: 9282 cells * (1/400)^2 lies on a rounding boundary of %.6f; the MPI
+    # log prints 0.058013 and the CUDA log 0.058012 (summation order)
+    assert "%.6f" % area0 == "%.6f" % a0_ref
+    assert "%.6f" % area1 == "%.6f" % a1_ref
+
+
+def test_sphere_deformation_64_log_known_answers(ls, cuda):
+    g = make_grid(3, 64, 64, 64, bc=(0, 0, 0))
+    o = Oracle(fma=True)
+    G0 = o.ic_sphere3d(g)
+    u, v, w = o.deformation_velocity(g, 0.0, 1.5)
+    p = ls.default_params(3, 64, reinit_interval=0, velocity=ls.VEL_DEFORM3D, vel_period=1.5)
+    s = ls.Solver(p)
+    dG, dG0 = _dev(cuda, G0), _dev(cuda, G0)
+    s.fill_ghosts(dG)
+    s.fill_ghosts(dG0)
+    _, vol0 = s.diag(dG0)
+    steps, t, dl = s.advance(dG, _dev(cuda, u), _dev(cuda, v), _dev(cuda, w), dt=0.001, t_final=1.5)
+    l2, vol1 = s.diag(dG, dG0)
+    assert steps == 1501 and dl < 1e-12  # sliver last step (log: dt 5.44e-14)
+    assert "%.6e" % l2 == "9.158634e-04"
+    assert "%.6f" % vol0 == "0.014165" and "%.6f" % vol1 == "0.014245"
+
+
+def test_config1_full_bitwise(ls, oracle, cuda):
+    """256^2 circle r=0.25, u=(0.5,0), S_L=0.1, CFL 0.5, 200 steps, HCR-2 every 10,
+    zero-gradient on all sides."""
+    p = ls.default_params(2, 256, s_l=0.1, reinit_interval=10, bc=(1, 1, 0))
+    G, u, v, _ = ls.init_case(p, ls.CASE_CIRCLE2D)
+    s = ls.Solver(p)
+    s.fill_ghosts(G)
+    Gh, uh, vh = G.cpu().numpy(), u.cpu().numpy(), v.cpu().numpy()
+    dt = ls.cfl_dt(p, 0.5, 0.5, 0.0, 0.0)
+    assert abs(dt - 0.5 * p.dx / 0.6) < 1e-18
+    steps, _, _ = s.advance(G, u, v, dt=dt, t_final=1e30, max_steps=200)
+    g = make_grid(2, 256, 256, bc=(1, 1, 0), s_l=0.1)
+    ref, n, _, _ = oracle.advance(g, Gh, uh, vh, dt=dt, t_final=1e30, max_steps=200, reinit_interval=10)
+    assert steps == n == 200
+    assert np.array_equal(G.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("n,steps", [(48, 60), (64, 25)])
+def test_config2_reduced_bitwise(ls, oracle, cuda, n, steps):
+    """Sphere r=0.2 in the swirl field, S_L=0.05, CFL 0.5, HCR-2 every 5, periodic."""
+    p = ls.default_params(3, n, s_l=0.05, reinit_interval=5)
+    G, u, v, w = ls.init_case(p, ls.CASE_SPHERE_SWIRL3D)
+    s = ls.Solver(p)
+    s.fill_ghosts(G)
+    host = [a.cpu().numpy() for a in (G, u, v, w)]
+    dt = 0.5 * p.dx / 1.25
+    s.advance(G, u, v, w, dt=dt, t_final=1e30, max_steps=steps)
+    g = make_grid(3, n, n, n, bc=(0, 0, 0), s_l=0.05)
+    ref, _, _, _ = oracle.advance(g, *host, dt=dt, t_final=1e30, max_steps=steps, reinit_interval=5)
+    assert np.array_equal(G.cpu().numpy(), ref)
+
+
+def test_config3_reduced_bitwise(ls, oracle, cuda):
+    """Plane x=0.3 with 8 seeded modes, u=(1,0,0)+16 solenoidal modes, S_L=0.08."""
+    n, steps = 40, 20
+    p = ls.default_params(3, n, s_l=0.08, reinit_interval=5)
+    G, u, v, w = ls.init_case(p, ls.CASE_PLANE_MODES3D, seed=3)
+    s = ls.Solver(p)
+    s.fill_ghosts(G)
+    host = [a.cpu().numpy() for a in (G, u, v, w)]
+    # the seeded velocity is divergence-free: check the discrete central divergence is O(h^2)
+    uh, vh, wh = host[1:]
+    I = (slice(3, 3 + n),) * 3
+    div = ((np.roll(uh, -1, 2) - np.roll(uh, 1, 2)) + (np.roll(vh, -1, 1) - np.roll(vh, 1, 1)) +
+           (np.roll(wh, -1, 0) - np.roll(wh, 1, 0))) / (2 * p.dx)
+    assert np.abs(div[I]).max() < 0.1 * (2 * math.pi * 4) ** 3 * p.dx ** 2
+    speed = (np.abs(uh) + np.abs(vh) + np.abs(wh)).max()
+    dt = 0.5 * p.dx / (speed + 0.08)
+    s.advance(G, u, v, w, dt=dt, t_final=1e30, max_steps=steps)
+    g = make_grid(3, n, n, n, bc=(0, 0, 0), s_l=0.08)
+    ref, _, _, _ = oracle.advance(g, *host, dt=dt, t_final=1e30, max_steps=steps, reinit_interval=5)
+    assert np.array_equal(G.cpu().numpy(), ref)
+
+
+def _roll_interior(cuda, a, shift, ng=3):
+    out = a.clone()
+    inner = a[ng:-ng, ng:-ng, ng:-ng]
+    out[ng:-ng, ng:-ng, ng:-ng] = cuda.roll(inner, shifts=shift, dims=(0, 1, 2))
+    return out
+
+
+@pytest.mark.parametrize("case,n,steps,s_l", [("c2", 256, 6, 0.05), ("c3", 512, 6, 0.08)])
+def test_full_size_translation_equivariance(ls, cuda, case, n, steps, s_l):
+    """With periodic boundaries every cell sees the same stencil values after a
+    whole-cell periodic shift, so shifting the inputs must shift the output
+    bit for bit (RK3 stages, ghost fills and HCR-2 included)."""
+    p = ls.default_params(3, n, s_l=s_l, reinit_interval=5)
+    cid = ls.CASE_SPHERE_SWIRL3D if case == "c2" else ls.CASE_PLANE_MODES3D
+    G, u, v, w = ls.init_case(p, cid, seed=3)
+    s = ls.Solver(p)
+    s.fill_ghosts(G)
+    shift = (5, -17, 33)
+    rolled = [_roll_interior(cuda, a, shift) for a in (G, u, v, w)]
+    for a in rolled:
+        s.fill_ghosts(a)
+    dt = 0.5 * p.dx / 1.6
+    s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=1, max_steps=steps)
+    s.advance(rolled[0], *rolled[1:], dt=dt, t_final=1e30, step0=1, max_steps=steps)
+    want = _roll_interior(cuda, G, shift)
+    assert s.check_finite(G) == 0
+    assert cuda.equal(want[3:-3, 3:-3, 3:-3], rolled[0][3:-3, 3:-3, 3:-3])
+    # determinism: rerun from the same inputs
+    G2, u2, v2, w2 = ls.init_case(p, cid, seed=3)
+    s.fill_ghosts(G2)
+    s.advance(G2, u2, v2, w2, dt=dt, t_final=1e30, step0=1, max_steps=steps)
+    assert cuda.equal(G, G2)
