@@ -168,10 +168,8 @@ def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=TRAFFIC_FILE):
         return out
     fr = per_kernel(_pmc_rows(fetch_dir), "FETCH_SIZE")
     wr = per_kernel(_pmc_rows(write_dir), "WRITE_SIZE")
-    copy_k = [k for k in fr if "copy" in k.lower() or "elementwise" in k.lower()]
-    stage_k = [k for k in fr if "stage" in k.lower() and "<3, 2>" in k.replace("ILi3ELi2E", "<3, 2>")]
-    if not stage_k:
-        stage_k = [k for k in fr if "k_stage" in k and "3ELi2" in k]
+    copy_k = [k for k in fr if "MulFunctor" in k]
+    stage_k = [k for k in fr if ("k_stage_%s<3, 2>" % mode) in k]
     if not copy_k or not stage_k:
         raise SystemExit("kernels not found in pmc output: %s" % list(fr))
     ck, sk = copy_k[0], stage_k[0]

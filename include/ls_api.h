@@ -146,6 +146,21 @@ int ls_rk3_stage(ls_solver_t* s, int32_t stage, const double* G_s, const double*
  * phi_0 = G and a ghost fill after each (reinitializeWithSwap). */
 int ls_reinit_hcr2(ls_solver_t* s, double* G, void* stream);
 
+/* One HCR-2 pseudo-time iteration phi_new = step(phi_n; phi_0) over all
+ * interior cells, then a ghost fill of phi_new (reinitStep +
+ * applyBoundaryConditions, reinitialization.h:200-263 and :344-355; 3D
+ * :136-193, :489-497).  Crossing flags and r~ are derived from phi_0 inside
+ * the kernel.  phi_new must not alias phi_n or phi_0. */
+int ls_hcr2_step(ls_solver_t* s, const double* phi_n, const double* phi_0, double* phi_new,
+                 void* stream);
+
+/* The HCR-2 set-up arrays of computeInterfaceCrossings (reinitialization.h:75-130,
+ * 3D :46-89) for callers that read them: r_tilde (padded doubles) and the
+ * crossing bit mask (padded int32: 1 x-, 2 x+, 4 y-, 8 y+, 16 z-, 32 z+) on
+ * interior cells; ghost entries are left untouched. */
+int ls_hcr2_crossings(ls_solver_t* s, const double* phi_0, double* r_tilde, int32_t* flag,
+                      void* stream);
+
 /* The reference time loop on G in place:
  *   while (t < t_final && step < step0 + max_steps):
  *     if t + dt > t_final: dt = t_final - t
@@ -198,6 +213,12 @@ int ls_write_G(const ls_params_t* p, const double* host_G, const char* path);
 #define LS_CASE_FLAME2D 6
 int ls_init_case(const ls_params_t* p, int32_t case_id, uint64_t seed, double period,
                  double* G, double* u, double* v, double* w, void* stream);
+
+/* The time-dependent deformation field of the 3D sphere test at time t,
+ * period `period`, on the whole padded arrays (updateDeformationVelocity,
+ * Local_code/level-set_MPI_3D/include/initial_conditions.h:232-280). */
+int ls_deformation_velocity(const ls_params_t* p, double t, double period, double* u, double* v,
+                            double* w, void* stream);
 
 /* Message for the last error on this thread (never NULL). */
 const char* ls_last_error(void);

@@ -18,7 +18,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblsamd.so")
+# LSAMD_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("LSAMD_LIB", os.path.join(HERE, "liblsamd.so"))
 
 LS_OK = 0
 LS_ERR_INVALID, LS_ERR_NOMEM, LS_ERR_DEVICE, LS_ERR_IO, LS_ERR_NONFINITE = -1, -2, -3, -4, -5
@@ -90,6 +91,8 @@ def lib():
     L.ls_rk3_step.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_double, vp]
     L.ls_rk3_stage.argtypes = [vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp, C.c_double, vp]
     L.ls_reinit_hcr2.argtypes = [vp, vp, vp]
+    L.ls_hcr2_step.argtypes = [vp, vp, vp, vp, vp]
+    L.ls_hcr2_crossings.argtypes = [vp, vp, vp, vp, vp]
     L.ls_advance.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int64, C.c_int64,
                              C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double), vp]
     L.ls_diag.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
@@ -192,6 +195,16 @@ class Solver:
         self._chk(G_s, G_n, out, u, v, w, sl)
         check(lib().ls_rk3_stage(self.h, k, _ptr(G_s), _ptr(G_n), _ptr(out), _ptr(u), _ptr(v),
                                  _ptr(w), _ptr(sl), dt, _stream(stream)))
+
+    def hcr2_step(self, phi_n, phi_0, phi_new, stream=None):
+        self._chk(phi_n, phi_0, phi_new)
+        check(lib().ls_hcr2_step(self.h, _ptr(phi_n), _ptr(phi_0), _ptr(phi_new), _stream(stream)))
+
+    def hcr2_crossings(self, phi_0, r_tilde, flag, stream=None):
+        self._chk(phi_0, r_tilde)
+        if flag.dtype.itemsize != 4 or tuple(flag.shape) != tuple(self.p.shape):
+            raise ValueError("flag must be an int32 field")
+        check(lib().ls_hcr2_crossings(self.h, _ptr(phi_0), _ptr(r_tilde), _ptr(flag), _stream(stream)))
 
     def reinit(self, G, stream=None):
         self._chk(G)

@@ -286,3 +286,12 @@ extern "C" int ls_init_case(const ls_params_t* p, int32_t case_id, uint64_t seed
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "case launch failed");
     return LS_OK;
 }
+
+extern "C" int ls_deformation_velocity(const ls_params_t* p, double t, double period, double* u,
+                                       double* v, double* w, void* stream) {
+    int rc = lsi::validate(p);
+    if (rc) return rc;
+    if (p->dim != 3 || !u || !v || !w) return lsi::set_error(LS_ERR_INVALID, "3D fields required");
+    if (!(period > 0.0)) return lsi::set_error(LS_ERR_INVALID, "period must be > 0");
+    return lsi::launch_deformation_velocity(*p, t, period, u, v, w, reinterpret_cast<hipStream_t>(stream));
+}

@@ -136,6 +136,33 @@ __global__ __launch_bounds__(256) void k_hcr2_exact(Geom g, const double* __rest
     pnew[c] = lsx::hcr2_cell<DIM>(g, pn + c, p0 + c);
 }
 
+// HCR-2 set-up arrays (computeInterfaceCrossings), for callers that read them.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_crossings(Geom g, const double* __restrict__ p0,
+                                                   double* __restrict__ rt, int32_t* __restrict__ flag) {
+    const int i = blockIdx.x * kTX + threadIdx.x;
+    const int j = blockIdx.y * kTY + threadIdx.y;
+    if (i >= g.nx || j >= g.ny) return;
+    const int k = DIM == 3 ? (int)blockIdx.z + g.ng : 0;
+    const int64_t c = (int64_t)k * g.sz + (int64_t)(j + g.ng) * g.sy + (i + g.ng);
+    const int64_t off[6] = {-1, 1, -g.sy, g.sy, -g.sz, g.sz};
+    constexpr int NB = DIM == 3 ? 6 : 4;
+    const double pc = p0[c];
+    double sum = 0.0;
+    int cnt = 0, f = 0;
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const double pq = p0[c + off[q]];
+        if (pc * pq < 0.0) {
+            sum += pq;
+            cnt++;
+            f |= 1 << q;
+        }
+    }
+    flag[c] = f;
+    rt[c] = (cnt > 0 && fabs(sum) > 1e-15) ? pc / sum : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Diagnostics: per-block partial sums in a fixed order, then one block sums the
 // partials in index order, so the result is deterministic run to run.
@@ -323,7 +350,8 @@ static int stage(const ls_solver* s, int k, const double* Gs, const double* Gn, 
                  hipStream_t st) {
     int rc = s->g.dim == 3 ? launch_stage<3>(s, k, Gs, Gn, out, u, v, w, sl, dt, st)
                            : launch_stage<2>(s, k, Gs, Gn, out, u, v, w, sl, dt, st);
-    if (rc) return rc;
+    if (rc < 0) return rc;
+    if (rc == 1) return LS_OK;  // the stage kernel wrote the ghost cells itself
     return launch_fill(s, out, st);
 }
 
@@ -454,6 +482,26 @@ int ls_rk3_stage(ls_solver_t* s, int32_t k, const double* G_s, const double* G_n
     if (out == G_s || out == G_n) return fail(LS_ERR_INVALID, "out must not alias an input");
     if (!std::isfinite(dt)) return fail(LS_ERR_INVALID, "dt must be finite");
     return stage(s, k, G_s, G_n, out, u, v, w, sl_field, dt, S(stream));
+}
+
+int ls_hcr2_step(ls_solver_t* s, const double* phi_n, const double* phi_0, double* phi_new,
+                 void* stream) {
+    if (!s || !phi_n || !phi_0 || !phi_new) return fail(LS_ERR_INVALID, "NULL argument");
+    if (phi_new == phi_n || phi_new == phi_0) return fail(LS_ERR_INVALID, "phi_new must not alias an input");
+    return hcr2_iter(s, phi_n, phi_0, phi_new, S(stream));
+}
+
+int ls_hcr2_crossings(ls_solver_t* s, const double* phi_0, double* r_tilde, int32_t* flag,
+                      void* stream) {
+    if (!s || !phi_0 || !r_tilde || !flag) return fail(LS_ERR_INVALID, "NULL argument");
+    const Geom& g = s->g;
+    dim3 grid = cell_grid(g), block(kTX, kTY, 1);
+    if (g.dim == 3)
+        hipLaunchKernelGGL((k_crossings<3>), grid, block, 0, S(stream), g, phi_0, r_tilde, flag);
+    else
+        hipLaunchKernelGGL((k_crossings<2>), grid, block, 0, S(stream), g, phi_0, r_tilde, flag);
+    LS_HIP(hipGetLastError());
+    return LS_OK;
 }
 
 int ls_reinit_hcr2(ls_solver_t* s, double* G, void* stream) {

@@ -33,3 +33,33 @@ def crossing_positions(G, axis, h):
     with np.errstate(divide="ignore", invalid="ignore"):
         off = np.where(cross, h * g0 / (g0 - g1), np.nan)
     return cross, off
+
+
+def front_parity(got, ref, h, rel_tol=1e-12, interior=None):
+    """Zero-level-set parity: along every axis the interpolated crossing
+    positions x_i + h G_i/(G_i - G_{i+1}) must agree within rel_tol * L per edge
+    (L = h * cells along the axis).  Edges whose sign pattern differs are
+    accepted only where the front sits on a node within rounding of zero in
+    both fields (|G| <= rel_tol * max|G|).  Returns the worst position error
+    relative to L."""
+    if interior is not None:
+        got, ref = got[interior], ref[interior]
+    scale = max(float(np.abs(ref).max()), 1e-300)
+    worst = 0.0
+    for ax in range(got.ndim):
+        hh = h[ax]
+        L = hh * got.shape[ax]
+        cg, og = crossing_positions(got, ax, hh)
+        cr, orr = crossing_positions(ref, ax, hh)
+        both = cg & cr
+        if both.any():
+            worst = max(worst, float(np.abs(og[both] - orr[both]).max()) / L)
+        diff = cg ^ cr
+        if diff.any():
+            a = np.moveaxis(got, ax, -1)
+            b = np.moveaxis(ref, ax, -1)
+            near = (np.minimum(np.abs(a[..., :-1]), np.abs(a[..., 1:])) <= rel_tol * scale) & \
+                   (np.minimum(np.abs(b[..., :-1]), np.abs(b[..., 1:])) <= rel_tol * scale)
+            assert near[diff].all(), "crossing pattern differs away from a node on axis %d" % ax
+    assert worst <= rel_tol, "front position differs by %.3e L" % worst
+    return worst
