@@ -314,10 +314,10 @@ __device__ __forceinline__ int image(int q, int n, int bc, int a) {
 // cells and the pointwise u, v, w, G_n (and S_L field) are issued one or two
 // planes ahead, so their latency overlaps the arithmetic of the current plane.
 #ifndef LS_ZDBUF
-#define LS_ZDBUF 1
+#define LS_ZDBUF 0
 #endif
 #ifndef LS_ZPF2
-#define LS_ZPF2 1
+#define LS_ZPF2 0
 #endif
 struct ZHalo {
     double own, h0, h1, xh;
@@ -507,6 +507,145 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LS_ZMARCH_W
     }
 }
 
+// ---------------------------------------------------------------------------
+// HCR-2 pseudo-time iteration, z-marching (3D).  Same structure as the stage
+// kernel with a 1-cell halo: the current planes of phi_n and phi_0 go through
+// LDS for the x/y neighbours, the z neighbours stay in registers, the next
+// plane is prefetched, and the ghost cells of phi_new are written by the
+// interior cells they copy.
+constexpr int HX = 64, HY = 4;
+
+struct HGeom {
+    ZGeom z;
+    double dx2, invdx, invdy, invdz, dtau, beta;
+};
+
+template <int W, int H>
+__device__ __forceinline__ void h_load_tile(const HGeom& hg, int64_t plane, int x0, int y0, int tid,
+                                            const double* __restrict__ F, double (&s)[H + 2][W + 2],
+                                            double own, bool active, int lx, int ly) {
+    const ZGeom& z = hg.z;
+    s[ly + 1][lx + 1] = own;
+    // halo ring: rows -1 and H (W cells each), columns -1 and W (H cells each)
+    for (int h = tid; h < 2 * W + 2 * H; h += W * H) {
+        int gx, gy, sx, syy;
+        if (h < 2 * W) {
+            sx = h % W + 1;
+            syy = h < W ? 0 : H + 1;
+        } else {
+            const int q = h - 2 * W;
+            syy = q % H + 1;
+            sx = q < H ? 0 : W + 1;
+        }
+        gx = x0 - 1 + sx;
+        gy = y0 - 1 + syy;
+        s[syy][sx] = (gx < z.nx + 3 && gy < z.ny + 3 && gx >= -3 && gy >= -3)
+                         ? F[plane + (int64_t)(gy + 3) * z.sy + (gx + 3)] : 0.0;
+    }
+    (void)active;
+}
+
+__global__ __launch_bounds__(256) void k_hcr2_zmarch(HGeom hg, const double* __restrict__ pn,
+                                                     const double* __restrict__ p0, double* __restrict__ pnew) {
+    __shared__ double sP[HY + 2][HX + 2];
+    __shared__ double sQ[HY + 2][HX + 2];
+    const ZGeom& z = hg.z;
+    const int64_t t = xcd_tile(blockIdx.x, z.nblocks);
+    if (t >= z.ntiles) return;  // block-uniform
+    const int tx = (int)(t % z.tiles_x);
+    const int64_t r = t / z.tiles_x;
+    const int ty = (int)(r % z.tiles_y);
+    const int ch = (int)(r / z.tiles_y);
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * HX + lx;
+    const int x0 = tx * HX, y0 = ty * HY;
+    const int i = x0 + lx, j = y0 + ly;
+    const bool active = i < z.nx && j < z.ny;
+    const bool inpad = i < z.nx + 3 && j < z.ny + 3;
+    const int kb = ch * z.kz;
+    const int ke = min(kb + z.kz, z.nz);
+    const int64_t sy = z.sy, sz = z.sz;
+    const int64_t col = (int64_t)(j + 3) * sy + (i + 3);
+    // column windows at k-1, k, k+1 (padded k index = k + 3)
+    double a0 = inpad ? pn[(int64_t)(kb + 2) * sz + col] : 0.0;
+    double a1 = inpad ? pn[(int64_t)(kb + 3) * sz + col] : 0.0;
+    double b0 = inpad ? p0[(int64_t)(kb + 2) * sz + col] : 0.0;
+    double b1 = inpad ? p0[(int64_t)(kb + 3) * sz + col] : 0.0;
+    double na = inpad ? pn[(int64_t)(kb + 4) * sz + col] : 0.0;
+    double nb = inpad ? p0[(int64_t)(kb + 4) * sz + col] : 0.0;
+    for (int k = kb; k < ke; k++) {
+        const int64_t plane = (int64_t)(k + 3) * sz;
+        const double a2 = na, b2 = nb;
+        if (k + 1 < ke && inpad) {
+            na = pn[plane + 2 * sz + col];
+            nb = p0[plane + 2 * sz + col];
+        }
+        __syncthreads();
+        h_load_tile<HX, HY>(hg, plane, x0, y0, tid, pn, sP, a1, active, lx, ly);
+        h_load_tile<HX, HY>(hg, plane, x0, y0, tid, p0, sQ, b1, active, lx, ly);
+        __syncthreads();
+        if (active) {
+            const double P[7] = {a1, sP[ly + 1][lx], sP[ly + 1][lx + 2], sP[ly][lx + 1], sP[ly + 2][lx + 1], a0, a2};
+            const double Q[7] = {b1, sQ[ly + 1][lx], sQ[ly + 1][lx + 2], sQ[ly][lx + 1], sQ[ly + 2][lx + 1], b0, b2};
+            // neighbours in flag order x-, x+, y-, y+, z-, z+ (reinitialization.h:46-89)
+            const double pc = Q[0];
+            double sum0 = 0.0;
+            int flag = 0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                if (pc * Q[q + 1] < 0.0) {
+                    sum0 += Q[q + 1];
+                    flag |= 1 << q;
+                }
+            }
+            const double phi = P[0];
+            const double S = pc * rsqrt_nr(fma(pc, pc, hg.dx2));
+            const double xm = (P[0] - P[1]) * hg.invdx, xp = (P[2] - P[0]) * hg.invdx;
+            const double ym = (P[0] - P[3]) * hg.invdy, yp = (P[4] - P[0]) * hg.invdy;
+            const double zm = (P[0] - P[5]) * hg.invdz, zp = (P[6] - P[0]) * hg.invdz;
+            const bool pos = S > 0.0;
+            const double ax = pos ? fmax(xm, 0.0) : fmin(xm, 0.0), bx = pos ? fmin(xp, 0.0) : fmax(xp, 0.0);
+            const double ay = pos ? fmax(ym, 0.0) : fmin(ym, 0.0), by = pos ? fmin(yp, 0.0) : fmax(yp, 0.0);
+            const double az = pos ? fmax(zm, 0.0) : fmin(zm, 0.0), bz = pos ? fmin(zp, 0.0) : fmax(zp, 0.0);
+            const double s2 = (fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by)) + fmax(az * az, bz * bz);
+            const double term = -S * (sqrt(s2) - 1.0);
+            double forcing = 0.0;
+            if (flag != 0) {
+                bool stable = (P[0] > 0.0) == (pc > 0.0);
+#pragma unroll
+                for (int q = 0; q < 6; q++)
+                    if ((flag >> q) & 1) stable = stable && ((P[q + 1] > 0.0) == (Q[q + 1] > 0.0));
+                if (stable) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 6; q++)
+                        if ((flag >> q) & 1) sum += P[q + 1];
+                    const double rt = fabs(sum0) > 1e-15 ? div_fast(pc, sum0) : 0.0;
+                    forcing = hg.beta * fma(rt, sum, -phi) * hg.invdx;
+                }
+            }
+            const double o = fma(hg.dtau, term + forcing, phi);
+            pnew[plane + col] = o;
+            if (z.fuse_ghosts && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3)) {
+                const int nxi = n_images(i, z.nx, z.bc[0]);
+                const int nyi = n_images(j, z.ny, z.bc[1]);
+                const int nzi = n_images(k, z.nz, z.bc[2]);
+                for (int a = 0; a < nzi; a++) {
+                    const int64_t pk = (int64_t)image(k, z.nz, z.bc[2], a) * sz;
+                    for (int b = 0; b < nyi; b++) {
+                        const int64_t pj = pk + (int64_t)image(j, z.ny, z.bc[1], b) * sy;
+                        for (int e = 0; e < nxi; e++)
+                            if (a | b | e) pnew[pj + image(i, z.nx, z.bc[0], e)] = o;
+                    }
+                }
+            }
+        }
+        a0 = a1;
+        a1 = a2;
+        b0 = b1;
+        b1 = b2;
+    }
+}
+
 static FastGeom make_fast(const Geom& g) {
     FastGeom f;
     f.g = g;
@@ -558,7 +697,7 @@ int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double
                         const double* u, const double* v, const double* w, const double* sl, double dt,
                         hipStream_t st) {
 #ifndef LS_ZMARCH_KZ
-#define LS_ZMARCH_KZ 64
+#define LS_ZMARCH_KZ 32
 #endif
     const int kz = g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16);
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
@@ -599,7 +738,24 @@ template int launch_stage_fast<2>(const Geom&, int, const double*, const double*
 template int launch_stage_fast<3>(const Geom&, int, const double*, const double*, double*, const double*,
                                   const double*, const double*, const double*, double, hipStream_t);
 
+// Returns 1 when the kernel also filled the ghost cells of pnew.
 int launch_hcr2_fast(const Geom& g, const double* pn, const double* p0, double* pnew, hipStream_t st) {
+    if (g.dim == 3) {
+        if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
+            return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
+        HGeom hg;
+        hg.z = make_z(g, g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16));
+        hg.dx2 = g.dx * g.dx;
+        hg.invdx = 1.0 / g.dx;
+        hg.invdy = 1.0 / g.dy;
+        hg.invdz = 1.0 / g.dz;
+        hg.dtau = g.dtau;
+        hg.beta = g.beta;
+        if (hg.z.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
+        hipLaunchKernelGGL(k_hcr2_zmarch, dim3((unsigned)hg.z.nblocks), dim3(HX, HY, 1), 0, st, hg, pn, p0, pnew);
+        if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast HCR-2 launch failed");
+        return hg.z.fuse_ghosts ? 1 : 0;
+    }
     const FastGeom f = make_fast(g);
     if (f.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
     const dim3 grid((unsigned)f.nblocks), block(kFX, kFY, 1);

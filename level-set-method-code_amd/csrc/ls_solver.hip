@@ -370,7 +370,8 @@ static int hcr2_iter(const ls_solver* s, const double* pn, const double* p0, dou
     const Geom& g = s->g;
     if (s->p.mode == LS_MODE_FAST) {
         int rc = lsf::launch_hcr2_fast(g, pn, p0, pnew, st);
-        if (rc) return rc;
+        if (rc < 0) return rc;
+        if (rc == 1) return LS_OK;  // the kernel wrote the ghost cells itself
     } else {
         dim3 grid = cell_grid(g), block(kTX, kTY, 1);
         if (g.dim == 3)
