@@ -16,6 +16,11 @@ Prints ONE JSON line (rank 0).  Options:
                            only (for rocprofv3 --pmc runs)
   --summarize-pmc F W      turn two rocprofv3 --pmc CSV directories (FETCH_SIZE,
                            WRITE_SIZE) into profiles/stage_kernel_traffic_512.json
+  --config batch           the batched-ensemble configuration instead: 96 2D
+                           flames of seeded sizes in {128,192,256,384,512}^2,
+                           S_L ~ U[0.05,0.2], 300 RK3 steps each, in one
+                           autotuned, hipGraph-replayed batch vs a serial
+                           per-instance loop (prints its own JSON line)
 """
 import argparse
 import csv
@@ -120,6 +125,82 @@ def cpu_baseline(n):
                       % (cores, n, n, el)}
 
 
+def ensemble(ls, count=96, seed=4, mode="fast"):
+    """The batched configuration: seeded member sizes, flame speeds and inputs."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    sizes = [128, 192, 256, 384, 512]
+    members, params = [], []
+    for q in range(count):
+        n = int(sizes[rng.integers(0, len(sizes))])
+        s_l = float(rng.uniform(0.05, 0.2))
+        p = ls.default_params(2, n, s_l=s_l, bc=(ls.BC_ZERO_GRAD, ls.BC_ZERO_GRAD, 0), reinit_interval=0,
+                              mode=ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT)
+        params.append(p)
+        members.append(dict(nx=n, ny=n, bc=(ls.BC_ZERO_GRAD, ls.BC_ZERO_GRAD), dx=p.dx, dy=p.dy, s_l=s_l,
+                            dt=0.0))
+    b0 = None
+    fields = []
+    for q, p in enumerate(params):
+        G, u, v, _ = ls.init_case(p, ls.CASE_FLAME2D, seed=1000 * seed + q)
+        speed = (u.abs() + v.abs()).max().item()
+        members[q]["dt"] = 0.5 * min(p.dx, p.dy) / (speed + p.s_l)
+        fields.append((G, u, v))
+    torch.cuda.synchronize()
+    return members, params, fields
+
+
+def bench_batch(ls, steps, mode):
+    import torch
+    members, params, fields = ensemble(ls, mode=mode)
+    b = ls.Batch(members, mode=ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT)
+    G, U, V = b.empty(), b.empty(), b.empty()
+    for q, (g, u, v) in enumerate(fields):
+        b.view(G, q).copy_(g)
+        b.view(U, q).copy_(u)
+        b.view(V, q).copy_(v)
+    b.fill_ghosts(G)
+    G0 = G.clone()
+    shape, probe_ms = b.autotune(G, U, V, probe_steps=3)
+    b.advance(G, U, V, steps, graph=True)  # capture + first replay (warm-up)
+    torch.cuda.synchronize()
+    G.copy_(G0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.advance(G, U, V, steps, graph=True)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    cells = sum(m["nx"] * m["ny"] for m in members)
+    # serial per-instance loop over the single-grid solver
+    solvers = [ls.Solver(p) for p in params]
+    singles = []
+    for q, (g, u, v) in enumerate(fields):
+        gg = g.clone()
+        solvers[q].fill_ghosts(gg)
+        singles.append(gg)
+    solvers[0].advance(singles[0].clone(), fields[0][1], fields[0][2], dt=members[0]["dt"], t_final=1e30,
+                       max_steps=2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in range(len(members)):
+        solvers[q].advance(singles[q], fields[q][1], fields[q][2], dt=members[q]["dt"], t_final=1e30,
+                           max_steps=steps)
+    torch.cuda.synchronize()
+    t_serial = time.perf_counter() - t0
+    same = all(torch.equal(b.view(G, q), singles[q]) for q in range(len(members)))
+    out = {"metric": "batched ensemble cell-updates/s (96 uneven 2D flames, 300 RK3 steps)",
+           "value": cells * steps / t_batch, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "96 2D flames, sizes in {128,192,256,384,512}^2 (seed 4), S_L~U[0.05,0.2], "
+                                  "circle/plane G0, uniform+8-mode velocity, zero-gradient edges",
+                      "members": len(members), "cells": cells, "mode": mode, "tile_shape": shape,
+                      "autotune_probe_ms_per_step": probe_ms},
+           "batched_s": t_batch, "serial_loop_s": t_serial, "speedup_vs_serial": t_serial / t_batch,
+           "serial_cell_updates_per_s": cells * steps / t_serial, "bitwise_equal_to_serial": bool(same)}
+    print(json.dumps(out), flush=True)
+
+
 def read_traffic(n, mode):
     if n != 512 or not os.path.exists(TRAFFIC_FILE):
         return None
@@ -206,6 +287,7 @@ def main():
     ap.add_argument("--profile-pass", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
+    ap.add_argument("--config", choices=["c3", "batch"], default="c3")
     a = ap.parse_args()
 
     if a.summarize_pmc:
@@ -222,6 +304,9 @@ def main():
     torch.cuda.set_device(0)
     if a.profile_pass:
         profile_pass(ls, a.n, a.mode)
+        return
+    if a.config == "batch":
+        bench_batch(ls, 300 if a.steps == 20 else a.steps, a.mode)
         return
 
     p, s, G, u, v, w, dt = setup(ls, a.n, a.mode)

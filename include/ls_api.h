@@ -220,6 +220,43 @@ int ls_init_case(const ls_params_t* p, int32_t case_id, uint64_t seed, double pe
 int ls_deformation_velocity(const ls_params_t* p, double t, double period, double* u, double* v,
                             double* w, void* stream);
 
+/* ---- Batched engine: many independent 2D flames of uneven sizes ----------
+ * Every member's padded field lives in one packed device buffer at element
+ * offset ls_batch_offset(b, m) (layout of one member as for a single grid).
+ * Each RK3 stage is one launch over all members' tiles (the kernel writes the
+ * ghost cells itself); ls_batch_advance captures nsteps steps into a hipGraph
+ * on first use and replays it while the arguments stay the same.  No HCR-2.
+ * The single-grid path it batches is rk3TimeStep (rk3.h:168-187) run as a
+ * fixed number of steps with a per-member dt. */
+typedef struct ls_batch_member {
+    int32_t nx, ny;     /* interior points, >= 6 */
+    int32_t bc[2];      /* LS_BC_PERIODIC or LS_BC_ZERO_GRAD for x and y */
+    double dx, dy;
+    double s_l;         /* flame speed of this member */
+    double dt;          /* time step of this member */
+} ls_batch_member_t;
+
+typedef struct ls_batch ls_batch_t;
+
+int ls_batch_create(const ls_batch_member_t* members, int32_t count, int32_t mode, ls_batch_t** out);
+int ls_batch_destroy(ls_batch_t* b);
+/* Total packed doubles, and the element offset of member m. */
+int64_t ls_batch_size(const ls_batch_t* b);
+int64_t ls_batch_offset(const ls_batch_t* b, int32_t member);
+/* Ghost fill of every member of a packed field. */
+int ls_batch_fill_ghosts(ls_batch_t* b, double* G, void* stream);
+/* nsteps RK3 steps of every member, G in place (packed).  use_graph != 0
+ * captures and replays a hipGraph.  Does not synchronise. */
+int ls_batch_advance(ls_batch_t* b, double* G, const double* u, const double* v, int32_t nsteps,
+                     int32_t use_graph, void* stream);
+/* Tile shapes 0..3 = 64x4, 64x2, 128x2, 32x8 threads; ls_batch_autotune times
+ * probe_steps steps of each on a scratch copy and keeps the fastest
+ * (*best_ms = its time per step).  Synchronises the stream. */
+int ls_batch_set_tile(ls_batch_t* b, int32_t shape);
+int32_t ls_batch_get_tile(const ls_batch_t* b);
+int ls_batch_autotune(ls_batch_t* b, const double* G, const double* u, const double* v,
+                      int32_t probe_steps, double* best_ms, void* stream);
+
 /* Message for the last error on this thread (never NULL). */
 const char* ls_last_error(void);
 

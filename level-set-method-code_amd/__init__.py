@@ -61,6 +61,12 @@ class Params(C.Structure):
         return self.nx * self.ny * (self.nz if self.dim == 3 else 1)
 
 
+class BatchMember(C.Structure):
+    """Mirror of ``ls_batch_member_t``."""
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("bc", C.c_int32 * 2),
+                ("dx", C.c_double), ("dy", C.c_double), ("s_l", C.c_double), ("dt", C.c_double)]
+
+
 class LSError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("ls error %d: %s" % (code, msg))
@@ -100,6 +106,19 @@ def lib():
     L.ls_write_G.argtypes = [P, vp, C.c_char_p]
     L.ls_init_case.argtypes = [P, C.c_int32, C.c_uint64, C.c_double, vp, vp, vp, vp, vp]
     L.ls_last_error.restype = C.c_char_p
+    BM = C.POINTER(BatchMember)
+    L.ls_batch_create.argtypes = [BM, C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.ls_batch_destroy.argtypes = [vp]
+    L.ls_batch_size.argtypes = [vp]
+    L.ls_batch_size.restype = C.c_int64
+    L.ls_batch_offset.argtypes = [vp, C.c_int32]
+    L.ls_batch_offset.restype = C.c_int64
+    L.ls_batch_fill_ghosts.argtypes = [vp, vp, vp]
+    L.ls_batch_advance.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, vp]
+    L.ls_batch_set_tile.argtypes = [vp, C.c_int32]
+    L.ls_batch_get_tile.argtypes = [vp]
+    L.ls_batch_get_tile.restype = C.c_int32
+    L.ls_batch_autotune.argtypes = [vp, vp, vp, vp, C.c_int32, C.POINTER(C.c_double), vp]
     _lib = L
     return L
 
@@ -252,3 +271,61 @@ def write_G(p, host_array, path):
     import numpy as np
     a = np.ascontiguousarray(host_array, dtype=np.float64)
     check(lib().ls_write_G(C.byref(p), C.c_void_p(a.ctypes.data), str(path).encode()))
+
+
+class Batch:
+    """Batched engine over packed float64 device buffers (see ls_api.h)."""
+
+    def __init__(self, members, mode=MODE_EXACT):
+        arr = (BatchMember * len(members))()
+        for q, m in enumerate(members):
+            arr[q].nx, arr[q].ny = m["nx"], m["ny"]
+            arr[q].bc[0], arr[q].bc[1] = m.get("bc", (BC_PERIODIC, BC_ZERO_GRAD))
+            arr[q].dx, arr[q].dy = m["dx"], m["dy"]
+            arr[q].s_l, arr[q].dt = m["s_l"], m["dt"]
+        self.members = members
+        self.h = C.c_void_p()
+        check(lib().ls_batch_create(arr, len(members), mode, C.byref(self.h)))
+        self.size = lib().ls_batch_size(self.h)
+        self.offsets = [lib().ls_batch_offset(self.h, q) for q in range(len(members))]
+
+    def close(self):
+        if self.h:
+            lib().ls_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def empty(self):
+        import torch
+        return torch.zeros(self.size, dtype=torch.float64, device="cuda")
+
+    def view(self, packed, q):
+        """Member q of a packed buffer as a (ny+6, nx+6) view."""
+        m = self.members[q]
+        n = (m["ny"] + 6) * (m["nx"] + 6)
+        o = self.offsets[q]
+        return packed[o:o + n].view(m["ny"] + 6, m["nx"] + 6)
+
+    def fill_ghosts(self, G, stream=None):
+        check(lib().ls_batch_fill_ghosts(self.h, _ptr(G), _stream(stream)))
+
+    def advance(self, G, u, v, nsteps, graph=True, stream=None):
+        check(lib().ls_batch_advance(self.h, _ptr(G), _ptr(u), _ptr(v), nsteps, 1 if graph else 0,
+                                     _stream(stream)))
+
+    def set_tile(self, shape):
+        check(lib().ls_batch_set_tile(self.h, shape))
+
+    def tile(self):
+        return lib().ls_batch_get_tile(self.h)
+
+    def autotune(self, G, u, v, probe_steps=3, stream=None):
+        ms = C.c_double()
+        check(lib().ls_batch_autotune(self.h, _ptr(G), _ptr(u), _ptr(v), probe_steps, C.byref(ms),
+                                      _stream(stream)))
+        return self.tile(), ms.value

@@ -1,0 +1,421 @@
+// ls_batch.hip -- many independent 2D flames of uneven sizes in one engine.
+//
+// All members' padded fields live in one packed device buffer (member m at
+// element offset off_m, 256-byte aligned).  One kernel launch per RK3 stage
+// covers every member: blocks walk a global tile list (member, tile x, tile y)
+// in a grid-stride loop, so small and large grids share the machine instead of
+// running one after another.  The stage kernel writes each output's ghost
+// cells itself (periodic / zero-gradient images), so a time step is three
+// launches.  ls_batch_advance captures the whole multi-step sequence into one
+// hipGraph and replays it, and ls_batch_autotune picks the tile shape that
+// runs fastest for the batch at hand.
+//
+// Per member the arithmetic is the single-grid solver's: the exact mode is
+// bitwise equal to the oracle, the fast mode within rounding (the same device
+// functions are used).  Steps are fixed-count with a per-member dt; there is
+// no HCR-2 in the batched engine.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "ls_api.h"
+#include "ls_exact.hpp"
+#include "ls_fast_math.hpp"
+#include "ls_internal.hpp"
+
+namespace {
+
+struct BMember {
+    int32_t nx, ny, nxt, bc0, bc1;
+    int32_t pad;
+    int64_t off;
+    double dx, dy, s_l, dt, eps;
+    double inv2dx, inv2dy, inv6dx, inv6dy;
+};
+
+struct BTile {
+    int32_t member, tx, ty, pad;
+};
+
+struct TileShape {
+    int tx, ty;
+};
+constexpr TileShape kShapes[] = {{64, 4}, {64, 2}, {128, 2}, {32, 8}};
+constexpr int kNumShapes = 4;
+
+__device__ __forceinline__ int b_nimg(int q, int n, int bc) {
+    if (bc == LS_BC_PERIODIC) return 1 + (q < 3) + (q >= n - 3);
+    return 1 + 3 * (q == 0) + 3 * (q == n - 1);
+}
+__device__ __forceinline__ int b_img(int q, int n, int bc, int a) {
+    if (a == 0) return q + 3;
+    if (bc == LS_BC_PERIODIC) return (a == 1 && q < 3) ? q + 3 + n : q + 3 - n;
+    return q == 0 ? a - 1 : n + 2 + a;
+}
+
+template <int MODE, int STAGE, int TX, int TY>
+__global__ __launch_bounds__(TX* TY) void k_batch_stage(const BMember* __restrict__ mem,
+                                                        const BTile* __restrict__ tiles, int ntiles,
+                                                        const double* __restrict__ Gs,
+                                                        const double* __restrict__ Gn,
+                                                        double* __restrict__ out,
+                                                        const double* __restrict__ u,
+                                                        const double* __restrict__ v) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const BTile tl = tiles[t];
+        const BMember m = mem[tl.member];
+        const int i = tl.tx * TX + (int)threadIdx.x;
+        const int j = tl.ty * TY + (int)threadIdx.y;
+        if (i >= m.nx || j >= m.ny) continue;
+        const int64_t sy = m.nxt;
+        const int64_t c = m.off + (int64_t)(j + 3) * sy + (i + 3);
+        const double* P = Gs + c;
+        double r;
+        if (MODE == LS_MODE_EXACT) {
+            lsx::Geom g;
+            g.sy = sy;
+            g.sz = 0;
+            g.dx = m.dx;
+            g.dy = m.dy;
+            g.eps = m.eps;
+            r = lsx::rhs_cell<2>(g, P, u[c], v[c], 0.0, m.s_l);
+        } else {
+            double ue = u[c], ve = v[c];
+            if (m.s_l > m.eps) {
+                const double gx = (P[1] - P[-1]) * m.inv2dx;
+                const double gy = (P[sy] - P[-sy]) * m.inv2dy;
+                const double k = m.s_l * lsf::rsqrt_nr(fma(gx, gx, gy * gy) + m.eps);
+                ue = fma(-k, gx, ue);
+                ve = fma(-k, gy, ve);
+            }
+            const double dgx = lsf::dweno_fast(P[-3], P[-2], P[-1], P[0], P[1], P[2], P[3], ue, m.inv6dx);
+            const double dgy = lsf::dweno_fast(P[-3 * sy], P[-2 * sy], P[-sy], P[0], P[sy], P[2 * sy], P[3 * sy],
+                                               ve, m.inv6dy);
+            r = -fma(ue, dgx, ve * dgy);
+        }
+        double o;
+        if (MODE == LS_MODE_EXACT) {
+            o = lsx::stage_combine<STAGE>(Gn[c], P[0], r, m.dt);
+        } else {
+            const double gn = Gn[c];
+            if (STAGE == 1) o = fma(m.dt, r, gn);
+            else if (STAGE == 2) o = fma(0.25 * m.dt, r, fma(0.25, P[0], 0.75 * gn));
+            else o = fma((2.0 / 3.0) * m.dt, r, fma(2.0 / 3.0, P[0], (1.0 / 3.0) * gn));
+        }
+        out[c] = o;
+        if (i < 3 || i >= m.nx - 3 || j < 3 || j >= m.ny - 3) {
+            const int nxi = b_nimg(i, m.nx, m.bc0), nyi = b_nimg(j, m.ny, m.bc1);
+            for (int b = 0; b < nyi; b++) {
+                const int64_t pj = m.off + (int64_t)b_img(j, m.ny, m.bc1, b) * sy;
+                for (int e = 0; e < nxi; e++)
+                    if (b | e) out[pj + b_img(i, m.nx, m.bc0, e)] = o;
+            }
+        }
+    }
+}
+
+// Ghost fill of every member (used once before stepping).
+__global__ __launch_bounds__(256) void k_batch_fill(const BMember* __restrict__ mem, double* __restrict__ G) {
+    const BMember m = mem[blockIdx.y];
+    const int nyt = m.ny + 6;
+    const int64_t n = (int64_t)m.nxt * nyt;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % m.nxt), j = (int)(t / m.nxt);
+        if (i >= 3 && i < m.nx + 3 && j >= 3 && j < m.ny + 3) continue;
+        int si = i, sj = j;
+        if (i < 3) si = m.bc0 == LS_BC_PERIODIC ? i + m.nx : 3;
+        if (i >= m.nx + 3) si = m.bc0 == LS_BC_PERIODIC ? i - m.nx : m.nx + 2;
+        if (j < 3) sj = m.bc1 == LS_BC_PERIODIC ? j + m.ny : 3;
+        if (j >= m.ny + 3) sj = m.bc1 == LS_BC_PERIODIC ? j - m.ny : m.ny + 2;
+        G[m.off + (int64_t)j * m.nxt + i] = G[m.off + (int64_t)sj * m.nxt + si];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        b[t] = a[t];
+}
+
+typedef void (*StageFn)(const BMember*, const BTile*, int, const double*, const double*, double*, const double*,
+                        const double*);
+
+template <int MODE, int TX, int TY>
+void pick3(StageFn* f) {
+    f[0] = k_batch_stage<MODE, 1, TX, TY>;
+    f[1] = k_batch_stage<MODE, 2, TX, TY>;
+    f[2] = k_batch_stage<MODE, 3, TX, TY>;
+}
+
+void stage_fns(int mode, int shape, StageFn* f) {
+    if (mode == LS_MODE_EXACT) {
+        switch (shape) {
+            case 0: pick3<LS_MODE_EXACT, 64, 4>(f); break;
+            case 1: pick3<LS_MODE_EXACT, 64, 2>(f); break;
+            case 2: pick3<LS_MODE_EXACT, 128, 2>(f); break;
+            default: pick3<LS_MODE_EXACT, 32, 8>(f); break;
+        }
+    } else {
+        switch (shape) {
+            case 0: pick3<LS_MODE_FAST, 64, 4>(f); break;
+            case 1: pick3<LS_MODE_FAST, 64, 2>(f); break;
+            case 2: pick3<LS_MODE_FAST, 128, 2>(f); break;
+            default: pick3<LS_MODE_FAST, 32, 8>(f); break;
+        }
+    }
+}
+
+}  // namespace
+
+struct ls_batch {
+    int mode;
+    int count;
+    int shape;
+    int64_t total;                 // packed doubles
+    std::vector<int64_t> offs;
+    std::vector<BMember> hmem;
+    BMember* dmem;
+    BTile* dtiles[kNumShapes];
+    int ntiles[kNumShapes];
+    double* buf[2];
+    // cached graph of the last captured sequence
+    hipGraphExec_t gexec;
+    hipGraph_t graph;
+    int g_steps, g_shape;
+    const void *g_G, *g_u, *g_v;
+    hipStream_t cap;               // capture stream owned by the batch (the null stream cannot be captured)
+};
+
+static int bfail(int code, const char* msg) { return lsi::set_error(code, msg); }
+
+static void drop_graph(ls_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// Enqueue nsteps RK3 steps on `st` (no synchronisation).
+static int enqueue_steps(ls_batch* b, double* G, const double* u, const double* v, int nsteps, hipStream_t st) {
+    StageFn f[3];
+    stage_fns(b->mode, b->shape, f);
+    const TileShape sh = kShapes[b->shape];
+    const int nt = b->ntiles[b->shape];
+    const unsigned grid = (unsigned)std::min(nt, 256 * 16);
+    const dim3 block(sh.tx, sh.ty, 1);
+    double* cur = G;
+    double* a = b->buf[0];
+    double* c = b->buf[1];
+    for (int s = 0; s < nsteps; s++) {
+        hipLaunchKernelGGL(f[0], dim3(grid), block, 0, st, b->dmem, b->dtiles[b->shape], nt, cur, cur, a, u, v);
+        hipLaunchKernelGGL(f[1], dim3(grid), block, 0, st, b->dmem, b->dtiles[b->shape], nt, a, cur, c, u, v);
+        hipLaunchKernelGGL(f[2], dim3(grid), block, 0, st, b->dmem, b->dtiles[b->shape], nt, c, cur, a, u, v);
+        double* old = cur;
+        cur = a;
+        a = old;
+    }
+    if (cur != G) {
+        hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, st, cur, G, b->total);
+    }
+    if (hipGetLastError() != hipSuccess) return bfail(LS_ERR_DEVICE, "batched stage launch failed");
+    return LS_OK;
+}
+
+extern "C" {
+
+int ls_batch_create(const ls_batch_member_t* members, int32_t count, int32_t mode, ls_batch_t** out) {
+    if (!out) return bfail(LS_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!members || count <= 0) return bfail(LS_ERR_INVALID, "no members");
+    if (mode != LS_MODE_EXACT && mode != LS_MODE_FAST) return bfail(LS_ERR_INVALID, "unknown mode");
+    ls_batch* b = new ls_batch();
+    b->mode = mode;
+    b->count = count;
+    b->shape = 0;
+    int64_t off = 0;
+    for (int m = 0; m < count; m++) {
+        const ls_batch_member_t& q = members[m];
+        if (q.nx < 6 || q.ny < 6 || !(q.dx > 0.0) || !(q.dy > 0.0) || !std::isfinite(q.dt) ||
+            (q.bc[0] != LS_BC_PERIODIC && q.bc[0] != LS_BC_ZERO_GRAD) ||
+            (q.bc[1] != LS_BC_PERIODIC && q.bc[1] != LS_BC_ZERO_GRAD)) {
+            delete b;
+            return bfail(LS_ERR_INVALID, "member needs nx, ny >= 6, positive spacing, finite dt, "
+                                         "periodic or zero-gradient boundaries");
+        }
+        BMember d;
+        std::memset(&d, 0, sizeof(d));
+        d.nx = q.nx;
+        d.ny = q.ny;
+        d.nxt = q.nx + 6;
+        d.bc0 = q.bc[0];
+        d.bc1 = q.bc[1];
+        d.off = off;
+        d.dx = q.dx;
+        d.dy = q.dy;
+        d.s_l = q.s_l;
+        d.dt = q.dt;
+        d.eps = 1.0e-10;
+        d.inv2dx = 1.0 / (2.0 * q.dx);
+        d.inv2dy = 1.0 / (2.0 * q.dy);
+        d.inv6dx = 1.0 / (6.0 * q.dx);
+        d.inv6dy = 1.0 / (6.0 * q.dy);
+        b->hmem.push_back(d);
+        b->offs.push_back(off);
+        const int64_t n = (int64_t)(q.nx + 6) * (q.ny + 6);
+        off += (n + 31) / 32 * 32;
+    }
+    b->total = off;
+    bool ok = hipMalloc((void**)&b->dmem, count * sizeof(BMember)) == hipSuccess &&
+              hipMemcpy(b->dmem, b->hmem.data(), count * sizeof(BMember), hipMemcpyHostToDevice) == hipSuccess;
+    for (int s = 0; ok && s < kNumShapes; s++) {
+        std::vector<BTile> tl;
+        // largest members first: they carry the most tiles and start earliest
+        std::vector<int> order(count);
+        for (int m = 0; m < count; m++) order[m] = m;
+        std::sort(order.begin(), order.end(), [&](int x, int y) {
+            return (int64_t)members[x].nx * members[x].ny > (int64_t)members[y].nx * members[y].ny;
+        });
+        for (int m : order) {
+            const int tx = (members[m].nx + kShapes[s].tx - 1) / kShapes[s].tx;
+            const int ty = (members[m].ny + kShapes[s].ty - 1) / kShapes[s].ty;
+            for (int y = 0; y < ty; y++)
+                for (int x = 0; x < tx; x++) tl.push_back(BTile{m, x, y, 0});
+        }
+        b->ntiles[s] = (int)tl.size();
+        ok = hipMalloc((void**)&b->dtiles[s], tl.size() * sizeof(BTile)) == hipSuccess &&
+             hipMemcpy(b->dtiles[s], tl.data(), tl.size() * sizeof(BTile), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    ok = ok && hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipMalloc((void**)&b->buf[0], b->total * sizeof(double)) == hipSuccess &&
+         hipMalloc((void**)&b->buf[1], b->total * sizeof(double)) == hipSuccess &&
+         hipMemset(b->buf[0], 0, b->total * sizeof(double)) == hipSuccess &&
+         hipMemset(b->buf[1], 0, b->total * sizeof(double)) == hipSuccess;
+    if (!ok) {
+        ls_batch_destroy(b);
+        return bfail(LS_ERR_NOMEM, "batch allocation failed");
+    }
+    *out = b;
+    return LS_OK;
+}
+
+int ls_batch_destroy(ls_batch_t* b) {
+    if (!b) return LS_OK;
+    drop_graph(b);
+    (void)hipFree(b->dmem);
+    for (int s = 0; s < kNumShapes; s++) (void)hipFree(b->dtiles[s]);
+    (void)hipFree(b->buf[0]);
+    (void)hipFree(b->buf[1]);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    delete b;
+    return LS_OK;
+}
+
+int64_t ls_batch_size(const ls_batch_t* b) { return b ? b->total : -1; }
+
+int64_t ls_batch_offset(const ls_batch_t* b, int32_t m) {
+    if (!b || m < 0 || m >= b->count) return -1;
+    return b->offs[m];
+}
+
+int ls_batch_set_tile(ls_batch_t* b, int32_t shape) {
+    if (!b || shape < 0 || shape >= kNumShapes) return bfail(LS_ERR_INVALID, "unknown tile shape");
+    if (shape != b->shape) drop_graph(b);
+    b->shape = shape;
+    return LS_OK;
+}
+
+int32_t ls_batch_get_tile(const ls_batch_t* b) { return b ? b->shape : -1; }
+
+int ls_batch_fill_ghosts(ls_batch_t* b, double* G, void* stream) {
+    if (!b || !G) return bfail(LS_ERR_INVALID, "NULL argument");
+    hipLaunchKernelGGL(k_batch_fill, dim3(64, b->count), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       b->dmem, G);
+    if (hipGetLastError() != hipSuccess) return bfail(LS_ERR_DEVICE, "batched ghost fill failed");
+    return LS_OK;
+}
+
+int ls_batch_advance(ls_batch_t* b, double* G, const double* u, const double* v, int32_t nsteps,
+                     int32_t use_graph, void* stream) {
+    if (!b || !G || !u || !v) return bfail(LS_ERR_INVALID, "NULL argument");
+    if (nsteps < 0) return bfail(LS_ERR_INVALID, "nsteps must be >= 0");
+    if (nsteps == 0) return LS_OK;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!use_graph) return enqueue_steps(b, G, u, v, nsteps, st);
+    const bool hit = b->gexec && b->g_steps == nsteps && b->g_shape == b->shape && b->g_G == G && b->g_u == u &&
+                     b->g_v == v;
+    if (!hit) {
+        drop_graph(b);
+        // Capture on the batch's own stream; the graph is then launched on
+        // the caller's stream, which may be the (uncapturable) null stream.
+        if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return bfail(LS_ERR_DEVICE, "hipStreamBeginCapture failed");
+        int rc = enqueue_steps(b, G, u, v, nsteps, b->cap);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->cap, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess || !g) return bfail(LS_ERR_DEVICE, "hipStreamEndCapture failed");
+        if (hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return bfail(LS_ERR_DEVICE, "hipGraphInstantiate failed");
+        }
+        b->graph = g;
+        b->g_steps = nsteps;
+        b->g_shape = b->shape;
+        b->g_G = G;
+        b->g_u = u;
+        b->g_v = v;
+    }
+    if (hipGraphLaunch(b->gexec, st) != hipSuccess) return bfail(LS_ERR_DEVICE, "hipGraphLaunch failed");
+    return LS_OK;
+}
+
+// Times `probe_steps` steps with every tile shape on a scratch copy of G and
+// keeps the fastest.  G, u and v are not modified.  Synchronises the stream.
+int ls_batch_autotune(ls_batch_t* b, const double* G, const double* u, const double* v, int32_t probe_steps,
+                      double* best_ms, void* stream) {
+    if (!b || !G || !u || !v || probe_steps <= 0) return bfail(LS_ERR_INVALID, "bad arguments");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    double* scratch = nullptr;
+    if (hipMalloc((void**)&scratch, b->total * sizeof(double)) != hipSuccess)
+        return bfail(LS_ERR_NOMEM, "autotune scratch allocation failed");
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int best = b->shape;
+    float best_t = 1e30f;
+    const int keep = b->shape;
+    int rc = LS_OK;
+    for (int s = 0; s < kNumShapes && rc == LS_OK; s++) {
+        b->shape = s;
+        hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, st, G, scratch, b->total);
+        rc = enqueue_steps(b, scratch, u, v, 1, st);  // warm-up
+        if (rc) break;
+        (void)hipEventRecord(e0, st);
+        rc = enqueue_steps(b, scratch, u, v, probe_steps, st);
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = bfail(LS_ERR_DEVICE, "autotune timing failed");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rc == LS_OK && ms < best_t) {
+            best_t = ms;
+            best = s;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    b->shape = keep;
+    if (rc) return rc;
+    ls_batch_set_tile(b, best);
+    if (best_ms) *best_ms = best_t / probe_steps;
+    return LS_OK;
+}
+
+}  // extern "C"
