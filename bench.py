@@ -21,6 +21,10 @@ Prints ONE JSON line (rank 0).  Options:
                            S_L ~ U[0.05,0.2], 300 RK3 steps each, in one
                            autotuned, hipGraph-replayed batch vs a serial
                            per-instance loop (prints its own JSON line)
+  --config adjoint         the adjoint configuration instead: flame-inversion
+                           gradients (128^3, 8^3 S_L nodes + 16 mode amplitudes,
+                           100 steps) for --sets seeded parameter sets vs
+                           central finite differences on the forward solver
 """
 import argparse
 import csv
@@ -201,6 +205,72 @@ def bench_batch(ls, steps, mode):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(ls, mode, nsets, fd_sample=16):
+    """The adjoint configuration: parameter gradients of the flame-inversion
+    loss for `nsets` seeded parameter sets vs central finite differences on
+    the plain forward solver (2 forward solves per parameter per gradient)."""
+    import numpy as np
+    import torch
+    m = ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT
+    prob = ls.FlameInversion(n=128, nc=8, nmodes=16, nsteps=100, reinit_interval=5, ckpt=10, mode=m)
+    sets = [prob.draw(100 + b) for b in range(nsets)]
+    prob.loss_and_grad(*sets[0])  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grads = []
+    for sl_c, amps in sets:
+        L, gc, ga = prob.loss_and_grad(sl_c, amps)
+        grads.append((L, gc, ga))
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / nsets
+    # one plain forward solve + loss: the unit of finite differences
+    reps = 8
+    prob.loss_plain(*sets[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        prob.loss_plain(*sets[r % nsets])
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    npar = prob.nc ** 3 + prob.nmodes
+    t_fd = 2 * npar * t_fwd
+    # agreement on a seeded sample of parameters of set 0 (h = 1e-6)
+    rng = np.random.default_rng(0)
+    n_s = fd_sample - 4
+    idx_s = sorted(rng.choice(prob.nc ** 3, n_s, replace=False).tolist())
+    idx_a = [0, 5, 10, 15]
+    sl_c, amps = sets[0]
+    h = 1e-6
+    fd = []
+    for q in idx_s:
+        sp, sm = sl_c.clone(), sl_c.clone()
+        sp[q] += h
+        sm[q] -= h
+        fd.append((prob.loss_plain(sp, amps) - prob.loss_plain(sm, amps)) / (2 * h))
+    for q in idx_a:
+        ap, am = amps.clone(), amps.clone()
+        ap[q] += h
+        am[q] -= h
+        fd.append((prob.loss_plain(sl_c, ap) - prob.loss_plain(sl_c, am)) / (2 * h))
+    fd = np.array(fd)
+    adj = np.concatenate([grads[0][1].cpu().numpy()[idx_s], grads[0][2].cpu().numpy()[idx_a]])
+    err = float(np.abs(adj - fd).max() / np.abs(fd).max())
+    out = {"metric": "adjoint vs FD grads/s (128³ sphere flame, 8³ S_L + 16 velocity modes, 100 RK3 steps)",
+           "value": 1.0 / t_adj, "unit": "gradients/s", "n_gpus": 1, "steps": prob.nsteps,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "3D 128^3 sphere flame r=0.2 in swirl + 16 seeded solenoidal modes, S_L on "
+                                  "8^3 coarse nodes (trilinear), loss sum (H_e(G_T)-H_e(G_target))^2, "
+                                  "e=1.5dx, HCR-2 every 5 steps, checkpoints every 10 steps",
+                      "parameter_sets": nsets, "parameters": npar, "mode": mode, "dt": prob.dt},
+           "adjoint_s_per_gradient": t_adj, "forward_solve_s": t_fwd,
+           "fd_s_per_gradient": t_fd, "fd_gradients_per_s": 1.0 / t_fd, "speedup_vs_fd": t_fd / t_adj,
+           "fd_sample": {"parameters": len(fd), "h": h, "max_rel_err": err},
+           "adjoint_workspace_bytes": prob.adj.workspace_bytes(),
+           "losses": [g[0] for g in grads[:4]]}
+    print(json.dumps(out), flush=True)
+    prob.close()
+
+
 def read_traffic(n, mode):
     if n != 512 or not os.path.exists(TRAFFIC_FILE):
         return None
@@ -287,7 +357,8 @@ def main():
     ap.add_argument("--profile-pass", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
-    ap.add_argument("--config", choices=["c3", "batch"], default="c3")
+    ap.add_argument("--config", choices=["c3", "batch", "adjoint"], default="c3")
+    ap.add_argument("--sets", type=int, default=32)
     a = ap.parse_args()
 
     if a.summarize_pmc:
@@ -307,6 +378,9 @@ def main():
         return
     if a.config == "batch":
         bench_batch(ls, 300 if a.steps == 20 else a.steps, a.mode)
+        return
+    if a.config == "adjoint":
+        bench_adjoint(ls, a.mode, a.sets)
         return
 
     p, s, G, u, v, w, dt = setup(ls, a.n, a.mode)

@@ -257,6 +257,54 @@ int32_t ls_batch_get_tile(const ls_batch_t* b);
 int ls_batch_autotune(ls_batch_t* b, const double* G, const double* u, const double* v,
                       int32_t probe_steps, double* best_ms, void* stream);
 
+/* ---- Discrete adjoint of the 3D solver ----------------------------------
+ * No counterpart in the reference, which has no derivatives; the adjoint
+ * differentiates the same discrete operators that ls_rk3_stage and
+ * ls_hcr2_step apply (rk3.h:31-125, weno5.h:28-248, reinitialization.h:46-193,
+ * boundary.h:250-261, all under Local_code/level-set_MPI_3D/include).
+ *
+ * Loss L = sum over interior cells of (H_e(G_T) - H_e(G_target))^2, with G_T
+ * the state after nsteps RK3 steps from G0 (HCR-2 after steps ri, 2ri, ...
+ * as in ls_advance with step0 = 0) and the smoothed Heaviside
+ * H_e(x) = 0 (x < -e), 1 (x > e), (1 + x/e + sin(pi x/e)/pi)/2 otherwise.
+ *
+ * ls_adjoint_create binds a context to a 3D solver (periodic or zero-gradient
+ * boundaries) and allocates its workspace: 26 padded fields, plus one per
+ * checkpoint (every ckpt steps), ckpt + 1 segment states, ckpt
+ * pre-reinitialisation states and reinit_iters - 1 HCR-2 iterates. */
+typedef struct ls_adjoint ls_adjoint_t;
+int ls_adjoint_create(ls_solver_t* s, int32_t nsteps, int32_t ckpt, ls_adjoint_t** out);
+int ls_adjoint_destroy(ls_adjoint_t* a);
+/* Device bytes held by the context. */
+int64_t ls_adjoint_workspace_bytes(const ls_adjoint_t* a);
+/* L for a given final state G (no time stepping).  Synchronises the stream. */
+int ls_adjoint_loss(ls_solver_t* s, const double* G, const double* G_target, double eps_h,
+                    double* loss, void* stream);
+/* Forward sweep with checkpoints, then the reverse sweep.  Writes *loss and,
+ * when gu/gv/gw are given, dL/du, dL/dv, dL/dw (interior cells), dL/dS_L
+ * (gsl, per cell; the sum over cells when sl_field is NULL and the scalar
+ * s_l is used) and dL/dG0 (gG0, ghosts folded into the interior).  All
+ * gradient pointers NULL: loss only.  sl_field as in ls_rk3_stage.
+ * Synchronises the stream. */
+int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const double* v,
+                   const double* w, const double* sl_field, const double* G_target,
+                   double eps_h, double dt, double* loss, double* gG0, double* gu, double* gv,
+                   double* gw, double* gsl, void* stream);
+
+/* Parameter maps of the flame-inversion problem (no counterpart in the
+ * reference).  S_L(x) is the trilinear interpolant of nc^3 coarse node values
+ * (device, x fastest) over the interior box; the velocity is the 3D swirl
+ * field of LS_CASE_SPHERE_SWIRL3D plus sum_q amps[q] * a_q sin(2 pi k_q.x + ph_q)
+ * over nmodes (<= 16) solenoidal unit modes drawn from mode_seed.  Either
+ * output group may be NULL.  ls_flame_project applies the transposed maps to
+ * the per-cell adjoints (interior cells) and synchronises the stream. */
+int ls_flame_fields(const ls_params_t* p, uint64_t mode_seed, int32_t nc, const double* sl_coarse,
+                    int32_t nmodes, const double* amps, double* sl_field, double* u, double* v,
+                    double* w, void* stream);
+int ls_flame_project(const ls_params_t* p, uint64_t mode_seed, int32_t nc, int32_t nmodes,
+                     const double* gsl, const double* gu, const double* gv, const double* gw,
+                     double* g_coarse, double* g_amps, void* stream);
+
 /* Message for the last error on this thread (never NULL). */
 const char* ls_last_error(void);
 

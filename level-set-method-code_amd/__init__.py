@@ -119,6 +119,15 @@ def lib():
     L.ls_batch_get_tile.argtypes = [vp]
     L.ls_batch_get_tile.restype = C.c_int32
     L.ls_batch_autotune.argtypes = [vp, vp, vp, vp, C.c_int32, C.POINTER(C.c_double), vp]
+    L.ls_adjoint_create.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.ls_adjoint_destroy.argtypes = [vp]
+    L.ls_adjoint_workspace_bytes.argtypes = [vp]
+    L.ls_adjoint_workspace_bytes.restype = C.c_int64
+    L.ls_adjoint_loss.argtypes = [vp, vp, vp, C.c_double, C.POINTER(C.c_double), vp]
+    L.ls_adjoint_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
+                                 C.POINTER(C.c_double), vp, vp, vp, vp, vp, vp]
+    L.ls_flame_fields.argtypes = [P, C.c_uint64, C.c_int32, vp, C.c_int32, vp, vp, vp, vp, vp, vp]
+    L.ls_flame_project.argtypes = [P, C.c_uint64, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -329,3 +338,194 @@ class Batch:
         check(lib().ls_batch_autotune(self.h, _ptr(G), _ptr(u), _ptr(v), probe_steps, C.byref(ms),
                                       _stream(stream)))
         return self.tile(), ms.value
+
+
+# ---------------------------------------------------------------------------
+# Discrete adjoint (3D)
+
+
+class Adjoint:
+    """Owns an ``ls_adjoint_t`` bound to a 3D :class:`Solver`: loss
+    sum (H_e(G_T) - H_e(G_target))^2 after ``nsteps`` RK3 steps (HCR-2 as in
+    the solver's params) and its gradient, checkpointed every ``ckpt`` steps."""
+
+    def __init__(self, solver, nsteps, ckpt=10):
+        self.solver = solver
+        self.nsteps = nsteps
+        self.h = C.c_void_p()
+        check(lib().ls_adjoint_create(solver.h, nsteps, ckpt, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ls_adjoint_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def workspace_bytes(self):
+        return lib().ls_adjoint_workspace_bytes(self.h)
+
+    def run(self, G0, u, v, w, G_target, eps_h, dt, sl=None, grad=True, out=None, stream=None):
+        """Returns (loss, grads); grads maps G0, u, v, w, sl to device fields
+        (None when grad is False).  ``out`` may supply the gradient tensors."""
+        s = self.solver
+        s._chk(G0, u, v, w, sl, G_target)
+        loss = C.c_double()
+        g = None
+        if grad:
+            g = out if out is not None else {k: s.empty() for k in ("G0", "u", "v", "w", "sl")}
+            s._chk(*g.values())
+        gp = (lambda k: _ptr(g[k]) if g is not None else None)
+        check(lib().ls_adjoint_run(self.h, _ptr(G0), _ptr(u), _ptr(v), _ptr(w), _ptr(sl),
+                                   _ptr(G_target), eps_h, dt, C.byref(loss), gp("G0"), gp("u"),
+                                   gp("v"), gp("w"), gp("sl"), _stream(stream)))
+        return loss.value, g
+
+
+def adjoint_loss(solver, G, G_target, eps_h, stream=None):
+    """The adjoint's loss for a given final field (no stepping)."""
+    solver._chk(G, G_target)
+    loss = C.c_double()
+    check(lib().ls_adjoint_loss(solver.h, _ptr(G), _ptr(G_target), eps_h, C.byref(loss),
+                                _stream(stream)))
+    return loss.value
+
+
+def flame_fields(p, mode_seed, sl_coarse=None, amps=None, sl=None, u=None, v=None, w=None,
+                 stream=None):
+    """S_L field from nc^3 coarse values (trilinear) and/or the velocity
+    swirl + sum amps[q] * mode_q, written into the given padded tensors."""
+    nc = round(sl_coarse.numel() ** (1.0 / 3.0)) if sl_coarse is not None else 2
+    if sl_coarse is not None and nc ** 3 != sl_coarse.numel():
+        raise ValueError("sl_coarse must hold nc^3 values")
+    nm = amps.numel() if amps is not None else 0
+    check(lib().ls_flame_fields(C.byref(p), mode_seed, nc, _ptr(sl_coarse), nm, _ptr(amps),
+                                _ptr(sl), _ptr(u), _ptr(v), _ptr(w), _stream(stream)))
+
+
+def flame_project(p, mode_seed, nc, nmodes, gsl, gu, gv, gw, stream=None):
+    """Transposed parameter maps: (dL/d sl_coarse [nc^3], dL/d amps [nmodes])."""
+    import torch
+    gc = torch.zeros(nc ** 3, dtype=torch.float64, device="cuda")
+    ga = torch.zeros(max(nmodes, 1), dtype=torch.float64, device="cuda")
+    check(lib().ls_flame_project(C.byref(p), mode_seed, nc, nmodes, _ptr(gsl), _ptr(gu), _ptr(gv),
+                                 _ptr(gw), _ptr(gc), _ptr(ga), _stream(stream)))
+    return gc, ga[:nmodes]
+
+
+class FlameInversion:
+    """Front-matching problem of the adjoint workload: a sphere flame
+    (r = 0.2 at the centre) on an n^3 grid in the swirl field plus
+    ``nmodes`` seeded solenoidal modes; parameters are S_L on an nc^3 coarse
+    grid (trilinear) and the mode amplitudes.  The target front comes from a
+    forward run with ``theta_true``; dt is fixed from the true fields' maxima
+    (CFL ``cfl``) so that the loss is a function of the parameters only."""
+
+    def __init__(self, n=128, nc=8, nmodes=16, nsteps=100, reinit_interval=5, ckpt=10,
+                 mode=MODE_EXACT, mode_seed=2024, true_seed=0, cfl=0.5, eps_factor=1.5):
+        import torch
+        self.p = default_params(3, n, reinit_interval=reinit_interval, mode=mode)
+        self.n, self.nc, self.nmodes, self.nsteps = n, nc, nmodes, nsteps
+        self.mode_seed = mode_seed
+        self.eps_h = eps_factor * self.p.dx
+        self.solver = Solver(self.p)
+        self.adj = Adjoint(self.solver, nsteps, ckpt)
+        G0, _, _, _ = init_case(self.p, CASE_SPHERE_SWIRL3D)
+        self.solver.fill_ghosts(G0)
+        self.G0 = G0
+        self.sl_true, self.amps_true = self.draw(true_seed)
+        self.sl, self.u, self.v, self.w = (self.solver.empty() for _ in range(4))
+        self.fields(self.sl_true, self.amps_true)
+        I = self.p.interior
+        vmax = sum(float(t[I].abs().max()) for t in (self.u, self.v, self.w))
+        self.dt = cfl * self.p.dx / (vmax + float(self.sl_true.max()))
+        self.G_target = self.forward(self.sl_true, self.amps_true)
+        self.grads = {k: self.solver.empty() for k in ("G0", "u", "v", "w", "sl")}
+        torch.cuda.synchronize()
+
+    def draw(self, seed):
+        """Seeded parameter set: S_L nodes ~ U[0.04, 0.12], amplitudes ~ U[-0.1, 0.1]."""
+        import numpy as np
+        import torch
+        rng = np.random.default_rng(seed)
+        sl = torch.tensor(rng.uniform(0.04, 0.12, self.nc ** 3), dtype=torch.float64, device="cuda")
+        amps = torch.tensor(rng.uniform(-0.1, 0.1, self.nmodes), dtype=torch.float64, device="cuda")
+        return sl, amps
+
+    def fields(self, sl_coarse, amps):
+        flame_fields(self.p, self.mode_seed, sl_coarse.contiguous(), amps.contiguous(), self.sl,
+                     self.u, self.v, self.w)
+
+    def forward(self, sl_coarse, amps):
+        """G_T from the plain solver loop (ls_advance)."""
+        self.fields(sl_coarse, amps)
+        G = self.G0.clone()
+        steps, _, _ = self.solver.advance(G, self.u, self.v, self.w, sl=self.sl, dt=self.dt,
+                                          t_final=(self.nsteps + 0.5) * self.dt,
+                                          max_steps=self.nsteps)
+        assert steps == self.nsteps
+        return G
+
+    def loss_plain(self, sl_coarse, amps):
+        """Loss through the plain forward solver: the finite-difference path."""
+        return adjoint_loss(self.solver, self.forward(sl_coarse, amps), self.G_target, self.eps_h)
+
+    def loss(self, sl_coarse, amps):
+        """Loss through the adjoint's forward sweep (no reverse sweep)."""
+        self.fields(sl_coarse, amps)
+        L, _ = self.adj.run(self.G0, self.u, self.v, self.w, self.G_target, self.eps_h, self.dt,
+                            sl=self.sl, grad=False)
+        return L
+
+    def loss_and_grad(self, sl_coarse, amps):
+        """(loss, dL/d sl_coarse, dL/d amps)."""
+        self.fields(sl_coarse, amps)
+        L, g = self.adj.run(self.G0, self.u, self.v, self.w, self.G_target, self.eps_h, self.dt,
+                            sl=self.sl, grad=True, out=self.grads)
+        gc, ga = flame_project(self.p, self.mode_seed, self.nc, self.nmodes, g["sl"], g["u"],
+                               g["v"], g["w"])
+        return L, gc, ga
+
+    def close(self):
+        self.adj.close()
+        self.solver.close()
+
+
+def _flame_loss_function():
+    import torch
+
+    class FlameLoss(torch.autograd.Function):
+        """loss = FlameLoss.apply(sl_coarse, amps, problem): forward and
+        reverse sweeps run together in ``forward`` when a gradient is
+        needed; ``backward`` scales the stored parameter gradients."""
+
+        @staticmethod
+        def forward(ctx, sl_coarse, amps, problem):
+            if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
+                L, gc, ga = problem.loss_and_grad(sl_coarse.detach(), amps.detach())
+                ctx.save_for_backward(gc.view_as(sl_coarse), ga.view_as(amps))
+            else:
+                L = problem.loss(sl_coarse.detach(), amps.detach())
+            return torch.tensor(L, dtype=torch.float64, device=sl_coarse.device)
+
+        @staticmethod
+        def backward(ctx, gL):
+            gc, ga = ctx.saved_tensors
+            return gL * gc, gL * ga, None
+
+    return FlameLoss
+
+
+_FLAME_LOSS = None
+
+
+def flame_loss(sl_coarse, amps, problem):
+    """Differentiable loss of a :class:`FlameInversion` (torch.autograd)."""
+    global _FLAME_LOSS
+    if _FLAME_LOSS is None:
+        _FLAME_LOSS = _flame_loss_function()
+    return _FLAME_LOSS.apply(sl_coarse, amps, problem)

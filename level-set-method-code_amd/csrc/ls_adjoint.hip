@@ -1,0 +1,696 @@
+// ls_adjoint.hip -- discrete adjoint of the 3D WENO-5 / TVD-RK3 / HCR-2 solver.
+//
+// Loss  L = sum over interior cells of (H_e(G_T) - H_e(G_target))^2 with the
+// smoothed Heaviside H_e(x) = 0 (x < -e), 1 (x > e),
+// (1 + x/e + sin(pi x/e)/pi) / 2 otherwise.  ls_adjoint_run returns L and
+// dL/dG0, dL/du, dL/dv, dL/dw and dL/dS_L (per-cell fields) by reverse-mode
+// differentiation of exactly the discrete scheme the forward solver runs:
+//   * RK3 stage (Local_code/level-set_MPI_3D/include/rk3.h:31-125): the RHS
+//     -(u_eff . grad_w G) is differentiated through the WENO-5 faces
+//     (beta -> alpha -> omega -> face, weno5.h:28-81), the central gradient
+//     and the u_eff normalisation (rk3.h:31-81); the upwind choice, the sign
+//     of u_eff (weno5.h:90-196), is the forward one;
+//   * ghost fill (boundary.h:250-261): a ghost cell copies an interior cell,
+//     so its adjoint is added into that cell ("fold");
+//   * HCR-2 step (reinitialization.h:46-193): smoothed sign of phi_0,
+//     Godunov |grad phi| (weno5.h:202-248) with the forward's max/min picks,
+//     forcing beta (r~ sum phi_n - phi_n) / dx with r~ = phi_0 / sum phi_0; the
+//     crossing flags and the stability test are discrete and keep their
+//     forward values.
+// The forward trajectory is checkpointed every `ckpt` steps; during the
+// reverse sweep each segment is recomputed once by the forward kernels, which
+// are deterministic, so every branch above is re-derived from a state that is
+// bitwise the forward state.
+//
+// Each per-cell kernel writes the partial derivatives of its own output into
+// per-offset planes (P_q[c] = lambda_c * d out_c / d G[c + off_q]); a gather
+// kernel sums, at every target cell, the planes of the cells whose stencil
+// holds it.  Fixed summation order, no atomics: the gradient is
+// deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ls_api.h"
+#include "ls_exact.hpp"
+#include "ls_internal.hpp"
+
+namespace {
+
+constexpr int kAX = 64, kAY = 4;
+constexpr int kLossBlocks = 1024;
+constexpr double kPi = 3.14159265358979323846;
+
+struct AGeom {
+    int32_t nx, ny, nz;
+    int32_t bc[3];
+    int64_t sy, sz, n;
+    double dx, dy, dz, eps, dtau, beta;
+};
+
+__device__ __forceinline__ int64_t cell(const AGeom& g, int i, int j, int k) {
+    return (int64_t)(k + 3) * g.sz + (int64_t)(j + 3) * g.sy + (i + 3);
+}
+
+// Smoothness indicator of (a,b,c) and its gradient.
+__device__ __forceinline__ double beta_g(double a, double b, double c, double& da, double& db, double& dc) {
+    const double D = a - 2.0 * b + c, E = c - a;
+    da = (13.0 / 6.0) * D - 0.5 * E;
+    db = -(13.0 / 3.0) * D;
+    dc = (13.0 / 6.0) * D + 0.5 * E;
+    return (13.0 / 12.0) * D * D + 0.25 * E * E;
+}
+
+// Left-biased WENO-5 face of (v0..v4) with linear weights (d0,d1,d2), and its
+// gradient df[0..4].  The right-biased face of the reference is this on the
+// reversed stencil with weights (0.3, 0.6, 0.1) (weno5.h:60-81).
+__device__ __forceinline__ double face_grad(const double* v, double d0, double d1, double d2, double* df) {
+    double gb[3][3], b[3];
+    b[0] = beta_g(v[0], v[1], v[2], gb[0][0], gb[0][1], gb[0][2]);
+    b[1] = beta_g(v[1], v[2], v[3], gb[1][0], gb[1][1], gb[1][2]);
+    b[2] = beta_g(v[2], v[3], v[4], gb[2][0], gb[2][1], gb[2][2]);
+    const double d[3] = {d0, d1, d2};
+    double a[3], s = 0.0;
+    for (int k = 0; k < 3; k++) {
+        const double e = lsx::kWenoEps + b[k];
+        a[k] = d[k] / (e * e);
+        s += a[k];
+    }
+    const double p[3] = {(2.0 * v[0] - 7.0 * v[1] + 11.0 * v[2]) / 6.0, (-v[1] + 5.0 * v[2] + 2.0 * v[3]) / 6.0,
+                         (2.0 * v[2] + 5.0 * v[3] - v[4]) / 6.0};
+    double w[3], f = 0.0;
+    for (int k = 0; k < 3; k++) {
+        w[k] = a[k] / s;
+        f += w[k] * p[k];
+    }
+    df[0] = w[0] * (2.0 / 6.0);
+    df[1] = w[0] * (-7.0 / 6.0) + w[1] * (-1.0 / 6.0);
+    df[2] = w[0] * (11.0 / 6.0) + w[1] * (5.0 / 6.0) + w[2] * (2.0 / 6.0);
+    df[3] = w[1] * (2.0 / 6.0) + w[2] * (5.0 / 6.0);
+    df[4] = w[2] * (-1.0 / 6.0);
+    for (int k = 0; k < 3; k++) {
+        // d f / d beta_k = -2 w_k (p_k - f) / (eps + beta_k)
+        const double gk = -2.0 * w[k] * (p[k] - f) / (lsx::kWenoEps + b[k]);
+        df[k] += gk * gb[k][0];
+        df[k + 1] += gk * gb[k][1];
+        df[k + 2] += gk * gb[k][2];
+    }
+    return f;
+}
+
+// Upwind WENO-5 derivative along one axis from the stencil s7 = (m3..p3), and
+// its gradient P[0..6].
+__device__ __forceinline__ double dweno_grad(const double* s7, double vel, double h, double* P) {
+    double fa, fb, ga[5], gb[5];
+    for (int q = 0; q < 7; q++) P[q] = 0.0;
+    if (vel >= 0.0) {
+        fa = face_grad(s7 + 0, 0.1, 0.6, 0.3, ga);  // face i-1/2 from m3..p1
+        fb = face_grad(s7 + 1, 0.1, 0.6, 0.3, gb);  // face i+1/2 from m2..p2
+        for (int q = 0; q < 5; q++) {
+            P[q] -= ga[q] / h;
+            P[q + 1] += gb[q] / h;
+        }
+    } else {
+        const double ra[5] = {s7[5], s7[4], s7[3], s7[2], s7[1]};  // face i-1/2, mirrored m2..p2
+        const double rb[5] = {s7[6], s7[5], s7[4], s7[3], s7[2]};  // face i+1/2, mirrored m1..p3
+        fa = face_grad(ra, 0.3, 0.6, 0.1, ga);
+        fb = face_grad(rb, 0.3, 0.6, 0.1, gb);
+        for (int q = 0; q < 5; q++) {
+            P[5 - q] -= ga[q] / h;
+            P[6 - q] += gb[q] / h;
+        }
+    }
+    return (fb - fa) / h;
+}
+
+// Plane offsets.  19-point stencil: 0 centre, 1..6 x -3,-2,-1,+1,+2,+3,
+// 7..12 y, 13..18 z.  7-point stencil: 0 centre, x-, x+, y-, y+, z-, z+.
+__device__ __forceinline__ int64_t off19(const AGeom& g, int q) {
+    if (q == 0) return 0;
+    const int a = (q - 1) / 6, r = (q - 1) % 6;
+    const int o = r < 3 ? r - 3 : r - 2;
+    return a == 0 ? (int64_t)o : (a == 1 ? o * g.sy : o * g.sz);
+}
+__device__ __forceinline__ int64_t off7(const AGeom& g, int q) {
+    switch (q) {
+        case 0: return 0;
+        case 1: return -1;
+        case 2: return 1;
+        case 3: return -g.sy;
+        case 4: return g.sy;
+        case 5: return -g.sz;
+        default: return g.sz;
+    }
+}
+
+// Stage partials.  lam = cdt * lambda_out[c] (cdt = the stage's coefficient
+// of dt*RHS times dt); writes lam * dRHS_c/dG[c + off19(q)] to plane q and
+// adds the pointwise velocity and flame-speed adjoints.
+__global__ __launch_bounds__(256) void k_adj_stage(AGeom g, const double* __restrict__ Gs, const double* __restrict__ u,
+                                                   const double* __restrict__ v, const double* __restrict__ w,
+                                                   const double* __restrict__ sl, double sl0,
+                                                   const double* __restrict__ lam_out, double cdt,
+                                                   double* __restrict__ P, double* __restrict__ gu,
+                                                   double* __restrict__ gv, double* __restrict__ gw,
+                                                   double* __restrict__ gsl) {
+    const int i = blockIdx.x * kAX + threadIdx.x, j = blockIdx.y * kAY + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int64_t c = cell(g, i, j, k);
+    const double lam = cdt * lam_out[c];
+    const double* G = Gs + c;
+    const double s_l = sl ? sl[c] : sl0;
+    double gx = 0.0, gy = 0.0, gz = 0.0, m = 1.0;
+    const bool on = s_l > g.eps;
+    if (on) {
+        gx = (G[1] - G[-1]) / (2.0 * g.dx);
+        gy = (G[g.sy] - G[-g.sy]) / (2.0 * g.dy);
+        gz = (G[g.sz] - G[-g.sz]) / (2.0 * g.dz);
+        m = sqrt(gx * gx + gy * gy + gz * gz + g.eps);
+    }
+    const double ue = u[c] - s_l * gx / m, ve = v[c] - s_l * gy / m, we = w[c] - s_l * gz / m;
+    double s7[7], Px[7], Py[7], Pz[7];
+    for (int q = 0; q < 7; q++) s7[q] = G[q - 3];
+    const double dgx = dweno_grad(s7, ue, g.dx, Px);
+    for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sy];
+    const double dgy = dweno_grad(s7, ve, g.dy, Py);
+    for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sz];
+    const double dgz = dweno_grad(s7, we, g.dz, Pz);
+    double out[19];
+    out[0] = -(ue * Px[3] + ve * Py[3] + we * Pz[3]);
+    for (int q = 0; q < 6; q++) {
+        const int t = q < 3 ? q : q + 1;
+        out[1 + q] = -ue * Px[t];
+        out[7 + q] = -ve * Py[t];
+        out[13 + q] = -we * Pz[t];
+    }
+    if (on) {
+        // d RHS / d g_b = s_l (dg_b - g_b Q / m^2) / m with Q = sum_a dg_a g_a
+        const double Q = dgx * gx + dgy * gy + dgz * gz;
+        const double rx = s_l * (dgx - gx * Q / (m * m)) / m;
+        const double ry = s_l * (dgy - gy * Q / (m * m)) / m;
+        const double rz = s_l * (dgz - gz * Q / (m * m)) / m;
+        out[3] -= rx / (2.0 * g.dx);
+        out[4] += rx / (2.0 * g.dx);
+        out[9] -= ry / (2.0 * g.dy);
+        out[10] += ry / (2.0 * g.dy);
+        out[15] -= rz / (2.0 * g.dz);
+        out[16] += rz / (2.0 * g.dz);
+        if (gsl) gsl[c] += lam * (Q / m);
+    }
+    for (int q = 0; q < 19; q++) P[(int64_t)q * g.n + c] = lam * out[q];
+    gu[c] -= lam * dgx;
+    gv[c] -= lam * dgy;
+    gw[c] -= lam * dgz;
+}
+
+// target[p] = a * self[p] + sum_q P_q[p - off_q] (+ target[p] if accumulate)
+// over every padded cell p.  The planes are zero outside the interior, so a
+// source index that leaves the array or wraps a row contributes nothing.
+template <int NQ>
+__global__ __launch_bounds__(256) void k_gather(AGeom g, const double* __restrict__ P, const double* __restrict__ self,
+                                                double a, double* __restrict__ target, int accumulate) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n; p += (int64_t)gridDim.x * blockDim.x) {
+        double acc = self ? a * self[p] : 0.0;
+        for (int q = 0; q < NQ; q++) {
+            const int64_t src = p - (NQ == 19 ? off19(g, q) : off7(g, q));
+            if (src >= 0 && src < g.n) acc += P[(int64_t)q * g.n + src];
+        }
+        target[p] = accumulate ? target[p] + acc : acc;
+    }
+}
+
+// Number of padded positions that hold interior index q along an axis of n
+// points (itself plus its ghost images), and the a-th of them.  Periodic:
+// padded q+3 and, for q < 3, q+3+n, for q >= n-3, q+3-n.  Zero gradient:
+// the edge points also fill their three ghosts.
+__device__ __forceinline__ int n_img(int q, int n, int bc) {
+    if (bc == LS_BC_PERIODIC) return 1 + (q < 3) + (q >= n - 3);
+    return 1 + 3 * (q == 0) + 3 * (q == n - 1);
+}
+__device__ __forceinline__ int img_at(int q, int n, int bc, int a) {
+    if (a == 0) return q + 3;
+    if (bc == LS_BC_PERIODIC) return (a == 1 && q < 3) ? q + 3 + n : q + 3 - n;
+    return q == 0 ? a - 1 : n + 2 + a;
+}
+
+// Adjoint of a ghost fill: each interior cell near a face adds the adjoints of
+// the ghost cells that copy it and clears them.
+__global__ __launch_bounds__(256) void k_fold(AGeom g, double* __restrict__ lam) {
+    const int i = blockIdx.x * kAX + threadIdx.x, j = blockIdx.y * kAY + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int ni = n_img(i, g.nx, g.bc[0]), nj = n_img(j, g.ny, g.bc[1]), nk = n_img(k, g.nz, g.bc[2]);
+    if (ni * nj * nk == 1) return;
+    double acc = 0.0;
+    for (int c = 0; c < nk; c++)
+        for (int b = 0; b < nj; b++)
+            for (int a = 0; a < ni; a++) {
+                if (!(a | b | c)) continue;
+                const int64_t q = (int64_t)img_at(k, g.nz, g.bc[2], c) * g.sz +
+                                  (int64_t)img_at(j, g.ny, g.bc[1], b) * g.sy + img_at(i, g.nx, g.bc[0], a);
+                acc += lam[q];
+                lam[q] = 0.0;
+            }
+    lam[cell(g, i, j, k)] += acc;
+}
+
+// HCR-2 partials of one pseudo-time step phi_new = step(phi_n; phi_0):
+// Pn_q[c] = lam_c * d phi_new_c / d phi_n[c + off7(q)], P0_q likewise for phi_0.
+__global__ __launch_bounds__(256) void k_adj_hcr2(AGeom g, const double* __restrict__ pn, const double* __restrict__ p0,
+                                                  const double* __restrict__ lam_new, double* __restrict__ Pn,
+                                                  double* __restrict__ P0) {
+    const int i = blockIdx.x * kAX + threadIdx.x, j = blockIdx.y * kAY + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int64_t c = cell(g, i, j, k);
+    const double lam = lam_new[c];
+    double N[7], Z[7], dN[7], dZ[7];
+    for (int q = 0; q < 7; q++) {
+        N[q] = pn[c + off7(g, q)];
+        Z[q] = p0[c + off7(g, q)];
+        dN[q] = 0.0;
+        dZ[q] = 0.0;
+    }
+    const double pc = Z[0];
+    double sum0 = 0.0;
+    int flag = 0;
+    for (int q = 0; q < 6; q++)
+        if (pc * Z[q + 1] < 0.0) {
+            sum0 += Z[q + 1];
+            flag |= 1 << q;
+        }
+    const bool rt_on = flag != 0 && fabs(sum0) > 1e-15;
+    const double rt = rt_on ? pc / sum0 : 0.0;
+    const double den = pc * pc + g.dx * g.dx;
+    const double S = pc / sqrt(den);
+    const double dS = g.dx * g.dx / (den * sqrt(den));
+    // Godunov |grad phi_n|: s2 = sum over axes of max(A^2, B^2)
+    const double h[3] = {g.dx, g.dy, g.dz};
+    double s2 = 0.0, gs2[7];
+    for (int q = 0; q < 7; q++) gs2[q] = 0.0;
+    for (int a = 0; a < 3; a++) {
+        const int qm = 1 + 2 * a, qp = 2 + 2 * a;
+        const double Dm = (N[0] - N[qm]) / h[a], Dp = (N[qp] - N[0]) / h[a];
+        double A, B;
+        bool Aon, Bon;
+        if (S > 0.0) {
+            A = fmax(Dm, 0.0);
+            B = fmin(Dp, 0.0);
+            Aon = Dm > 0.0;
+            Bon = Dp < 0.0;
+        } else {
+            A = fmin(Dm, 0.0);
+            B = fmax(Dp, 0.0);
+            Aon = Dm < 0.0;
+            Bon = Dp > 0.0;
+        }
+        if (A * A >= B * B) {
+            s2 += A * A;
+            if (Aon) {
+                gs2[0] += 2.0 * A / h[a];
+                gs2[qm] -= 2.0 * A / h[a];
+            }
+        } else {
+            s2 += B * B;
+            if (Bon) {
+                gs2[qp] += 2.0 * B / h[a];
+                gs2[0] -= 2.0 * B / h[a];
+            }
+        }
+    }
+    const double gm = sqrt(s2);
+    // phi_new = phi_n + dtau (-S (gm - 1) + forcing)
+    dN[0] += 1.0;
+    if (gm > 0.0)
+        for (int q = 0; q < 7; q++) dN[q] -= g.dtau * S * gs2[q] / (2.0 * gm);
+    dZ[0] -= g.dtau * dS * (gm - 1.0);
+    if (flag != 0) {
+        bool stable = (N[0] > 0.0) == (pc > 0.0);
+        for (int q = 0; q < 6; q++)
+            if ((flag >> q) & 1) stable = stable && ((N[q + 1] > 0.0) == (Z[q + 1] > 0.0));
+        if (stable) {
+            double sumn = 0.0;
+            for (int q = 0; q < 6; q++)
+                if ((flag >> q) & 1) sumn += N[q + 1];
+            const double bd = g.beta / g.dx;
+            dN[0] -= g.dtau * bd;
+            for (int q = 0; q < 6; q++)
+                if ((flag >> q) & 1) dN[q + 1] += g.dtau * bd * rt;
+            if (rt_on) {
+                dZ[0] += g.dtau * bd * sumn / sum0;
+                for (int q = 0; q < 6; q++)
+                    if ((flag >> q) & 1) dZ[q + 1] -= g.dtau * bd * pc * sumn / (sum0 * sum0);
+            }
+        }
+    }
+    for (int q = 0; q < 7; q++) {
+        Pn[(int64_t)q * g.n + c] = lam * dN[q];
+        P0[(int64_t)q * g.n + c] = lam * dZ[q];
+    }
+}
+
+__device__ __forceinline__ double heaviside(double x, double e, double& d) {
+    if (x < -e) {
+        d = 0.0;
+        return 0.0;
+    }
+    if (x > e) {
+        d = 0.0;
+        return 1.0;
+    }
+    d = (1.0 + cos(kPi * x / e)) / (2.0 * e);
+    return 0.5 * (1.0 + x / e + sin(kPi * x / e) / kPi);
+}
+
+// Per-block partial loss sums over interior cells; lam (optional) receives
+// dL/dG on interior cells.
+__global__ __launch_bounds__(256) void k_loss(AGeom g, const double* __restrict__ G, const double* __restrict__ T,
+                                              double e, double* __restrict__ lam, double* __restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    const int64_t nc = (int64_t)g.nx * g.ny * g.nz;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nc; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % g.nx), j = (int)((t / g.nx) % g.ny), k = (int)(t / ((int64_t)g.nx * g.ny));
+        const int64_t c = cell(g, i, j, k);
+        double dG, dT;
+        const double r = heaviside(G[c], e, dG) - heaviside(T[c], e, dT);
+        acc += r * r;
+        if (lam) lam[c] = 2.0 * r * dG;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_loss_final(const double* __restrict__ part, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < kLossBlocks; b++) s += part[b];
+        *out = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_axpy(int64_t n, double a, const double* __restrict__ x,
+                                              double* __restrict__ y) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        y[p] += a * x[p];
+}
+
+AGeom make_ageom(const ls_params_t& p) {
+    AGeom g;
+    std::memset(&g, 0, sizeof(g));
+    g.nx = p.nx;
+    g.ny = p.ny;
+    g.nz = p.nz;
+    for (int a = 0; a < 3; a++) g.bc[a] = p.bc[a];
+    g.sy = p.nx + 6;
+    g.sz = g.sy * (p.ny + 6);
+    g.n = g.sz * (p.nz + 6);
+    g.dx = p.dx;
+    g.dy = p.dy;
+    g.dz = p.dz;
+    g.eps = p.eps;
+    g.dtau = p.reinit_dtau;
+    g.beta = p.reinit_beta;
+    return g;
+}
+
+}  // namespace
+
+struct ls_adjoint {
+    ls_solver_t* s = nullptr;
+    ls_params_t p;
+    AGeom g;
+    int nsteps = 0, ckpt = 0, nseg = 0, iters = 0;
+    size_t fb = 0;
+    double* P = nullptr;        // 19 partial planes (ghost regions stay zero)
+    double* lam = nullptr;      // adjoint of the current state
+    double* lam2 = nullptr;     // scratch adjoints
+    double* lam3 = nullptr;
+    double* acc = nullptr;      // adjoint of G_n while a step is reversed
+    double* G1 = nullptr;       // recomputed stage fields
+    double* G2 = nullptr;
+    double* part = nullptr;     // loss partials + result
+    double* loss_host = nullptr;
+    std::vector<double*> ck;    // checkpoints: state at the start of step m*ckpt
+    std::vector<double*> seg;   // states of one segment (ckpt + 1)
+    std::vector<double*> pre;   // pre-reinitialisation states of one segment
+    std::vector<double*> hit;   // HCR-2 iterates 1..iters-1 (hit[0] unused)
+    std::vector<double*> all;   // every device allocation
+};
+
+namespace {
+
+void release(ls_adjoint* a) {
+    for (double* q : a->all) (void)hipFree(q);
+    if (a->loss_host) (void)hipHostFree(a->loss_host);
+    delete a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ls_adjoint_create(ls_solver_t* s, int32_t nsteps, int32_t ckpt, ls_adjoint_t** out) {
+    if (!out) return lsi::set_error(LS_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!s) return lsi::set_error(LS_ERR_INVALID, "solver is NULL");
+    const ls_params_t& p = lsi::solver_params(s);
+    if (p.dim != 3) return lsi::set_error(LS_ERR_INVALID, "the adjoint covers the 3D solver");
+    for (int a = 0; a < 3; a++)
+        if (p.bc[a] != LS_BC_PERIODIC && p.bc[a] != LS_BC_ZERO_GRAD)
+            return lsi::set_error(LS_ERR_INVALID, "the adjoint supports periodic and zero-gradient boundaries");
+    if (p.nghost != 3) return lsi::set_error(LS_ERR_INVALID, "the adjoint needs 3 ghost layers");
+    if (p.nx < 6 || p.ny < 6 || p.nz < 6) return lsi::set_error(LS_ERR_INVALID, "the adjoint needs >= 6 points per axis");
+    if (nsteps < 1 || ckpt < 1) return lsi::set_error(LS_ERR_INVALID, "nsteps and ckpt must be >= 1");
+    ls_adjoint* a = new ls_adjoint();
+    a->s = s;
+    a->p = p;
+    a->g = make_ageom(p);
+    a->nsteps = nsteps;
+    a->ckpt = std::min(ckpt, nsteps);
+    a->nseg = (nsteps + a->ckpt - 1) / a->ckpt;
+    a->iters = (p.reinit_interval > 0) ? p.reinit_iters : 0;
+    a->fb = (size_t)a->g.n * sizeof(double);
+    auto field = [&](size_t count) -> double* {
+        double* d = nullptr;
+        if (hipMalloc((void**)&d, count * a->fb) != hipSuccess) return nullptr;
+        a->all.push_back(d);
+        return d;
+    };
+    bool ok = (a->P = field(19)) && (a->lam = field(1)) && (a->lam2 = field(1)) && (a->lam3 = field(1)) &&
+              (a->acc = field(1)) && (a->G1 = field(1)) && (a->G2 = field(1));
+    if (ok) {
+        ok = hipMalloc((void**)&a->part, (kLossBlocks + 1) * sizeof(double)) == hipSuccess;
+        if (ok) a->all.push_back(a->part);
+    }
+    for (int q = 0; ok && q < a->nseg; q++) ok = (a->ck.emplace_back(field(1)) != nullptr);
+    for (int q = 0; ok && q <= a->ckpt; q++) ok = (a->seg.emplace_back(field(1)) != nullptr);
+    for (int q = 0; ok && a->iters > 0 && q < a->ckpt; q++) ok = (a->pre.emplace_back(field(1)) != nullptr);
+    if (a->iters > 0) a->hit.push_back(nullptr);
+    for (int q = 1; ok && q < a->iters; q++) ok = (a->hit.emplace_back(field(1)) != nullptr);
+    if (ok) ok = hipHostMalloc((void**)&a->loss_host, sizeof(double), hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        release(a);
+        return lsi::set_error(LS_ERR_NOMEM, "adjoint workspace allocation failed");
+    }
+    // The planes are written on interior cells only; their ghost regions must
+    // read as zero in every gather.
+    if (hipMemset(a->P, 0, 19 * a->fb) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        release(a);
+        return lsi::set_error(LS_ERR_DEVICE, "adjoint workspace initialisation failed");
+    }
+    *out = a;
+    return LS_OK;
+}
+
+int ls_adjoint_destroy(ls_adjoint_t* a) {
+    if (a) release(a);
+    return LS_OK;
+}
+
+int64_t ls_adjoint_workspace_bytes(const ls_adjoint_t* a) {
+    if (!a) return -1;
+    // every allocation is one field except P (19 fields) and the loss partials
+    return (int64_t)(a->all.size() - 1 + 18) * (int64_t)a->fb + (kLossBlocks + 1) * (int64_t)sizeof(double);
+}
+
+int ls_adjoint_loss(ls_solver_t* s, const double* G, const double* G_target, double eps_h, double* loss,
+                    void* stream) {
+    if (!s || !G || !G_target || !loss) return lsi::set_error(LS_ERR_INVALID, "NULL argument");
+    const ls_params_t& p = lsi::solver_params(s);
+    if (p.dim != 3 || p.nghost != 3) return lsi::set_error(LS_ERR_INVALID, "the loss covers 3D fields with 3 ghosts");
+    if (!(eps_h > 0.0)) return lsi::set_error(LS_ERR_INVALID, "eps_h must be > 0");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const AGeom g = make_ageom(p);
+    double* part = nullptr;
+    if (hipMalloc((void**)&part, (kLossBlocks + 1) * sizeof(double)) != hipSuccess)
+        return lsi::set_error(LS_ERR_NOMEM, "loss scratch allocation failed");
+    hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, st, g, G, G_target, eps_h, (double*)nullptr, part);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, st, part, part + kLossBlocks);
+    double h = 0.0;
+    const bool ok = hipGetLastError() == hipSuccess &&
+                    hipMemcpyAsync(&h, part + kLossBlocks, sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+                    hipStreamSynchronize(st) == hipSuccess;
+    (void)hipFree(part);
+    if (!ok) return lsi::set_error(LS_ERR_DEVICE, "loss evaluation failed");
+    *loss = h;
+    return LS_OK;
+}
+
+int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const double* v, const double* w,
+                   const double* sl_field, const double* G_target, double eps_h, double dt, double* loss,
+                   double* gG0, double* gu, double* gv, double* gw, double* gsl, void* stream) {
+    if (!a || !G0 || !u || !v || !w || !G_target || !loss) return lsi::set_error(LS_ERR_INVALID, "NULL argument");
+    if (!(eps_h > 0.0) || !(dt > 0.0)) return lsi::set_error(LS_ERR_INVALID, "eps_h and dt must be > 0");
+    const bool grad = gu || gv || gw || gsl || gG0;
+    if (grad && (!gu || !gv || !gw)) return lsi::set_error(LS_ERR_INVALID, "gu, gv and gw are required for a gradient");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    ls_solver_t* s = a->s;
+    const AGeom& g = a->g;
+    const size_t fb = a->fb;
+    const int ri = a->p.reinit_interval;
+    const int iters = a->iters;
+    const int ckpt = a->ckpt;
+    const double sl0 = a->p.s_l;
+    const dim3 cgrid((g.nx + kAX - 1) / kAX, (g.ny + kAY - 1) / kAY, g.nz), cblk(kAX, kAY, 1);
+    const dim3 lgrid(1024), lblk(256);
+    int rc = LS_OK;
+#define ADJ_HIP(call)                                                          \
+    do {                                                                       \
+        if ((call) != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, #call); \
+    } while (0)
+#define ADJ_RC(call)                  \
+    do {                              \
+        if ((rc = (call))) return rc; \
+    } while (0)
+
+    auto reinit_after = [&](int n) { return iters > 0 && n > 0 && n % ri == 0; };
+
+    // Forward step n from `from` into `to`; with reinitialisation the RK3
+    // result goes to `prestate` and the HCR-2 iterates through hit[].
+    auto fwd_step = [&](const double* from, double* to, double* prestate, int n) -> int {
+        ADJ_RC(ls_rk3_stage(s, 1, from, from, a->G1, u, v, w, sl_field, dt, st));
+        ADJ_RC(ls_rk3_stage(s, 2, a->G1, from, a->G2, u, v, w, sl_field, dt, st));
+        if (!reinit_after(n)) return ls_rk3_stage(s, 3, a->G2, from, to, u, v, w, sl_field, dt, st);
+        ADJ_RC(ls_rk3_stage(s, 3, a->G2, from, prestate, u, v, w, sl_field, dt, st));
+        const double* in = prestate;
+        for (int it = 0; it < iters; it++) {
+            double* o = it + 1 == iters ? to : a->hit[it + 1];
+            ADJ_RC(ls_hcr2_step(s, in, prestate, o, st));
+            in = o;
+        }
+        return LS_OK;
+    };
+
+    // Forward sweep: checkpoints and G_T.
+    ADJ_HIP(hipMemcpyAsync(a->ck[0], G0, fb, hipMemcpyDeviceToDevice, st));
+    ADJ_RC(ls_fill_ghosts(s, a->ck[0], st));
+    const double* cur = a->ck[0];
+    for (int n = 0; n < a->nsteps; n++) {
+        const bool to_ck = (n + 1) % ckpt == 0 && n + 1 < a->nsteps;
+        double* nxt = to_ck ? a->ck[(n + 1) / ckpt] : (cur == a->seg[0] ? a->seg[1] : a->seg[0]);
+        ADJ_RC(fwd_step(cur, nxt, iters > 0 ? a->pre[0] : nullptr, n));
+        cur = nxt;
+    }
+    double* lamT = grad ? a->lam : nullptr;
+    if (grad) ADJ_HIP(hipMemsetAsync(a->lam, 0, fb, st));
+    hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, st, g, cur, G_target, eps_h, lamT, a->part);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, st, a->part, a->part + kLossBlocks);
+    ADJ_HIP(hipGetLastError());
+    ADJ_HIP(hipMemcpyAsync(a->loss_host, a->part + kLossBlocks, sizeof(double), hipMemcpyDeviceToHost, st));
+    if (!grad) {
+        ADJ_HIP(hipStreamSynchronize(st));
+        *loss = *a->loss_host;
+        return LS_OK;
+    }
+    ADJ_HIP(hipMemsetAsync(gu, 0, fb, st));
+    ADJ_HIP(hipMemsetAsync(gv, 0, fb, st));
+    ADJ_HIP(hipMemsetAsync(gw, 0, fb, st));
+    if (gsl) ADJ_HIP(hipMemsetAsync(gsl, 0, fb, st));
+
+    // Reverse of one stage out = A G_n + B G_s + C dt RHS(G_s) followed by a
+    // ghost fill.  lam_in: adjoint of out (its ghosts are folded here).  The
+    // adjoint of G_s goes to lam_s (overwritten); A lam_in is added to acc.
+    // Stage 1 has G_s = G_n: everything goes to acc.
+    auto stage_vjp = [&](int stage, const double* Gs, double* lam_in, double* lam_s) -> int {
+        const double A = stage == 1 ? 1.0 : (stage == 2 ? 0.75 : 1.0 / 3.0);
+        const double B = stage == 1 ? 0.0 : (stage == 2 ? 0.25 : 2.0 / 3.0);
+        const double C = stage == 1 ? 1.0 : (stage == 2 ? 0.25 : 2.0 / 3.0);
+        hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, lam_in);
+        hipLaunchKernelGGL(k_adj_stage, cgrid, cblk, 0, st, g, Gs, u, v, w, sl_field, sl0, lam_in, C * dt, a->P, gu,
+                           gv, gw, gsl);
+        if (stage == 1) {
+            hipLaunchKernelGGL((k_gather<19>), lgrid, lblk, 0, st, g, a->P, lam_in, A + B, a->acc, 1);
+        } else {
+            hipLaunchKernelGGL((k_gather<19>), lgrid, lblk, 0, st, g, a->P, lam_in, B, lam_s, 0);
+            hipLaunchKernelGGL(k_axpy, lgrid, lblk, 0, st, g.n, A, lam_in, a->acc);
+        }
+        ADJ_HIP(hipGetLastError());
+        return LS_OK;
+    };
+
+    // Reverse of a reinitialisation: a->lam holds the adjoint of its output;
+    // on return it holds the adjoint of phi_0 (the pre-reinit state).
+    auto hcr2_vjp = [&](const double* phi0) -> int {
+        ADJ_HIP(hipMemsetAsync(a->lam2, 0, fb, st));  // adjoint of phi_0 through the r~ and sign terms
+        for (int it = iters - 1; it >= 0; it--) {
+            const double* in = it == 0 ? phi0 : a->hit[it];
+            hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, a->lam);
+            hipLaunchKernelGGL(k_adj_hcr2, cgrid, cblk, 0, st, g, in, phi0, a->lam, a->P, a->P + 7 * g.n);
+            hipLaunchKernelGGL((k_gather<7>), lgrid, lblk, 0, st, g, a->P + 7 * g.n, (const double*)nullptr, 0.0,
+                               a->lam2, 1);
+            hipLaunchKernelGGL((k_gather<7>), lgrid, lblk, 0, st, g, a->P, (const double*)nullptr, 0.0, a->lam3, 0);
+            std::swap(a->lam, a->lam3);
+        }
+        // the first iterate is phi_0 itself
+        hipLaunchKernelGGL(k_axpy, lgrid, lblk, 0, st, g.n, 1.0, a->lam2, a->lam);
+        ADJ_HIP(hipGetLastError());
+        return LS_OK;
+    };
+
+    // Reverse sweep, one checkpoint segment at a time.
+    for (int sg = a->nseg - 1; sg >= 0; sg--) {
+        const int n0 = sg * ckpt, n1 = std::min(a->nsteps, n0 + ckpt);
+        const double* start = a->ck[sg];
+        for (int n = n0; n < n1; n++) {
+            const double* from = n == n0 ? start : a->seg[n - n0];
+            ADJ_RC(fwd_step(from, a->seg[n - n0 + 1], iters > 0 ? a->pre[n - n0] : nullptr, n));
+        }
+        for (int n = n1 - 1; n >= n0; n--) {
+            const int q = n - n0;
+            const double* Gn = q == 0 ? start : a->seg[q];
+            if (reinit_after(n)) {
+                const double* in = a->pre[q];
+                for (int it = 1; it < iters; it++) {
+                    ADJ_RC(ls_hcr2_step(s, in, a->pre[q], a->hit[it], st));
+                    in = a->hit[it];
+                }
+                ADJ_RC(hcr2_vjp(a->pre[q]));
+            }
+            ADJ_RC(ls_rk3_stage(s, 1, Gn, Gn, a->G1, u, v, w, sl_field, dt, st));
+            ADJ_RC(ls_rk3_stage(s, 2, a->G1, Gn, a->G2, u, v, w, sl_field, dt, st));
+            ADJ_HIP(hipMemsetAsync(a->acc, 0, fb, st));
+            ADJ_RC(stage_vjp(3, a->G2, a->lam, a->lam2));   // lam2: adjoint of G2
+            ADJ_RC(stage_vjp(2, a->G1, a->lam2, a->lam3));  // lam3: adjoint of G1
+            ADJ_RC(stage_vjp(1, Gn, a->lam3, nullptr));
+            std::swap(a->lam, a->acc);  // adjoint of G_n
+        }
+    }
+    // G0's ghosts were filled from its interior before the first step.
+    hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, a->lam);
+    ADJ_HIP(hipGetLastError());
+    if (gG0) ADJ_HIP(hipMemcpyAsync(gG0, a->lam, fb, hipMemcpyDeviceToDevice, st));
+    ADJ_HIP(hipStreamSynchronize(st));
+    *loss = *a->loss_host;
+    return LS_OK;
+#undef ADJ_HIP
+#undef ADJ_RC
+}
+
+}  // extern "C"

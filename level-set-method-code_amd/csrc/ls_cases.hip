@@ -216,6 +216,199 @@ void draw_solenoidal(SplitMix64& rng, int dim, double k[3], double a[3]) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Flame-inversion parameter maps.  The flame speed is the trilinear
+// interpolant of nc^3 coarse node values spread uniformly over the interior
+// box [x0, x0 + (nx-1) dx] x ...; the velocity is the swirl field of the
+// sphere test plus sum_q amp_q * a_q sin(2 pi k_q . x + ph_q) over seeded
+// solenoidal unit modes.  Both maps are linear in the parameters, so their
+// adjoints are the transposed sums below.
+struct FlameModes {
+    int n;
+    double k[kMaxModes][3], a[kMaxModes][3], ph[kMaxModes];
+};
+
+struct Coarse {
+    int nc;
+    double h[3];  // coarse node spacing per axis
+};
+
+__device__ __forceinline__ void trilin(double s, int nc, int& i0, double& f) {
+    s = fmin(fmax(s, 0.0), (double)(nc - 1));
+    int i = (int)floor(s);
+    if (i > nc - 2) i = nc - 2;
+    i0 = i;
+    f = s - i;
+}
+
+__device__ __forceinline__ void swirl(double x, double y, double z, double& u, double& v, double& w) {
+    u = sin(kPi * x) * cos(kPi * y);
+    v = -cos(kPi * x) * sin(kPi * y);
+    w = 0.2 * sin(2.0 * kPi * z);
+}
+
+__global__ __launch_bounds__(256) void k_flame_fields(Coord c, FlameModes m, Coarse cg,
+                                                      const double* __restrict__ slc,
+                                                      const double* __restrict__ amps,
+                                                      double* __restrict__ sl, double* __restrict__ u,
+                                                      double* __restrict__ v, double* __restrict__ w,
+                                                      int64_t ntot) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot) return;
+    const int i = (int)(t % c.nxt);
+    const int j = (int)((t / c.nxt) % c.nyt);
+    const int k = (int)(t / ((int64_t)c.nxt * c.nyt));
+    double x, y, z;
+    coord(c, i, j, k, x, y, z);
+    if (sl) {
+        int ix, iy, iz;
+        double fx, fy, fz;
+        trilin((x - c.x0) / cg.h[0], cg.nc, ix, fx);
+        trilin((y - c.y0) / cg.h[1], cg.nc, iy, fy);
+        trilin((z - c.z0) / cg.h[2], cg.nc, iz, fz);
+        const int nc = cg.nc;
+        double acc = 0.0;
+        for (int dz = 0; dz < 2; dz++)
+            for (int dy = 0; dy < 2; dy++)
+                for (int dx = 0; dx < 2; dx++) {
+                    const double wgt = (dx ? fx : 1.0 - fx) * (dy ? fy : 1.0 - fy) * (dz ? fz : 1.0 - fz);
+                    acc += wgt * slc[((iz + dz) * nc + (iy + dy)) * nc + (ix + dx)];
+                }
+        sl[t] = acc;
+    }
+    if (u) {
+        double uu, vv, ww;
+        swirl(x, y, z, uu, vv, ww);
+        for (int q = 0; q < m.n; q++) {
+            const double s = amps[q] * sin(2.0 * kPi * (m.k[q][0] * x + m.k[q][1] * y + m.k[q][2] * z) + m.ph[q]);
+            uu += s * m.a[q][0];
+            vv += s * m.a[q][1];
+            ww += s * m.a[q][2];
+        }
+        u[t] = uu;
+        v[t] = vv;
+        w[t] = ww;
+    }
+}
+
+// g_coarse[node] = sum over interior cells of weight(node, cell) * gsl[cell];
+// one block per coarse node walks the cells whose interpolation stencil can
+// hold the node, in a fixed order.
+__global__ __launch_bounds__(256) void k_flame_project_sl(Coord c, int nx, int ny, int nz, Coarse cg,
+                                                          const double* __restrict__ gsl,
+                                                          double* __restrict__ gc) {
+    __shared__ double red[256];
+    const int nc = cg.nc;
+    const int node = blockIdx.x;
+    const int I = node % nc, J = (node / nc) % nc, K = node / (nc * nc);
+    int lo[3], hi[3];
+    const int nn[3] = {nx, ny, nz};
+    const int ID[3] = {I, J, K};
+    const double d[3] = {c.dx, c.dy, c.dz};
+    for (int a = 0; a < 3; a++) {
+        lo[a] = (int)floor((ID[a] - 1) * cg.h[a] / d[a]) - 1;
+        hi[a] = (int)ceil((ID[a] + 1) * cg.h[a] / d[a]) + 1;
+        lo[a] = lo[a] < 0 ? 0 : lo[a];
+        hi[a] = hi[a] > nn[a] - 1 ? nn[a] - 1 : hi[a];
+    }
+    const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+    const int64_t cnt = (int64_t)ex * ey * ez;
+    double acc = 0.0;
+    for (int64_t q = threadIdx.x; q < cnt; q += blockDim.x) {
+        const int i = lo[0] + (int)(q % ex), j = lo[1] + (int)((q / ex) % ey), k = lo[2] + (int)(q / ((int64_t)ex * ey));
+        double x, y, z;
+        coord(c, i + c.ng, j + c.ng, k + c.ng, x, y, z);
+        int ix, iy, iz;
+        double fx, fy, fz;
+        trilin((x - c.x0) / cg.h[0], nc, ix, fx);
+        trilin((y - c.y0) / cg.h[1], nc, iy, fy);
+        trilin((z - c.z0) / cg.h[2], nc, iz, fz);
+        const int dx = I - ix, dy = J - iy, dz = K - iz;
+        if (dx < 0 || dx > 1 || dy < 0 || dy > 1 || dz < 0 || dz > 1) continue;
+        const double wgt = (dx ? fx : 1.0 - fx) * (dy ? fy : 1.0 - fy) * (dz ? fz : 1.0 - fz);
+        acc += wgt * gsl[((int64_t)(k + c.ng) * c.nyt + (j + c.ng)) * c.nxt + (i + c.ng)];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gc[node] = red[0];
+}
+
+constexpr int kProjBlocks = 512;
+
+// Per-block partial sums over interior cells of mode_q(x) . (gu, gv, gw).
+__global__ __launch_bounds__(256) void k_flame_project_modes(Coord c, int nx, int ny, int nz, FlameModes m,
+                                                             const double* __restrict__ gu,
+                                                             const double* __restrict__ gv,
+                                                             const double* __restrict__ gw,
+                                                             double* __restrict__ part) {
+    __shared__ double red[kMaxModes][256];
+    double acc[kMaxModes];
+    for (int q = 0; q < kMaxModes; q++) acc[q] = 0.0;
+    const int64_t ncell = (int64_t)nx * ny * nz;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+        const int64_t p = ((int64_t)(k + c.ng) * c.nyt + (j + c.ng)) * c.nxt + (i + c.ng);
+        double x, y, z;
+        coord(c, i + c.ng, j + c.ng, k + c.ng, x, y, z);
+        const double a = gu[p], b = gv[p], e = gw[p];
+        for (int q = 0; q < m.n; q++) {
+            const double s = sin(2.0 * kPi * (m.k[q][0] * x + m.k[q][1] * y + m.k[q][2] * z) + m.ph[q]);
+            acc[q] += s * (m.a[q][0] * a + m.a[q][1] * b + m.a[q][2] * e);
+        }
+    }
+    for (int q = 0; q < m.n; q++) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int q = 0; q < m.n; q++) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < m.n) part[blockIdx.x * kMaxModes + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_flame_project_final(int n, const double* __restrict__ part, double* __restrict__ ga) {
+    const int q = threadIdx.x;
+    if (q >= n) return;
+    double s = 0.0;
+    for (int b = 0; b < kProjBlocks; b++) s += part[b * kMaxModes + q];
+    ga[q] = s;
+}
+
+FlameModes draw_flame_modes(uint64_t seed, int n) {
+    FlameModes m;
+    std::memset(&m, 0, sizeof(m));
+    m.n = n;
+    SplitMix64 rng{seed};
+    for (int q = 0; q < n; q++) {
+        draw_solenoidal(rng, 3, m.k[q], m.a[q]);
+        m.ph[q] = 2.0 * kPi * rng.uniform();
+    }
+    return m;
+}
+
+int flame_check(const ls_params_t* p, int32_t nc, int32_t nmodes) {
+    int rc = lsi::validate(p);
+    if (rc) return rc;
+    if (p->dim != 3) return lsi::set_error(LS_ERR_INVALID, "the flame maps are 3D");
+    if (nc < 2 || nc > 64) return lsi::set_error(LS_ERR_INVALID, "coarse grid size must be in [2, 64]");
+    if (nmodes < 0 || nmodes > kMaxModes) return lsi::set_error(LS_ERR_INVALID, "at most 16 velocity modes");
+    return LS_OK;
+}
+
+Coarse make_coarse(const ls_params_t& p, int nc) {
+    Coarse cg;
+    cg.nc = nc;
+    cg.h[0] = (p.nx - 1) * p.dx / (nc - 1);
+    cg.h[1] = (p.ny - 1) * p.dy / (nc - 1);
+    cg.h[2] = (p.nz - 1) * p.dz / (nc - 1);
+    return cg;
+}
+
 }  // namespace
 
 namespace lsi {
@@ -294,4 +487,49 @@ extern "C" int ls_deformation_velocity(const ls_params_t* p, double t, double pe
     if (p->dim != 3 || !u || !v || !w) return lsi::set_error(LS_ERR_INVALID, "3D fields required");
     if (!(period > 0.0)) return lsi::set_error(LS_ERR_INVALID, "period must be > 0");
     return lsi::launch_deformation_velocity(*p, t, period, u, v, w, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int ls_flame_fields(const ls_params_t* p, uint64_t mode_seed, int32_t nc, const double* sl_coarse,
+                               int32_t nmodes, const double* amps, double* sl_field, double* u, double* v,
+                               double* w, void* stream) {
+    int rc = flame_check(p, nc, nmodes);
+    if (rc) return rc;
+    if (sl_field && !sl_coarse) return lsi::set_error(LS_ERR_INVALID, "sl_coarse is NULL");
+    if (u && (!v || !w || (nmodes > 0 && !amps))) return lsi::set_error(LS_ERR_INVALID, "NULL velocity argument");
+    if (!sl_field && !u) return LS_OK;
+    const Coord c = make_coord(*p);
+    const int64_t n = (int64_t)c.nxt * c.nyt * c.nzt;
+    hipLaunchKernelGGL(k_flame_fields, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), c, draw_flame_modes(mode_seed, nmodes),
+                       make_coarse(*p, nc), sl_coarse, amps, sl_field, u, v, w, n);
+    if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "flame field launch failed");
+    return LS_OK;
+}
+
+extern "C" int ls_flame_project(const ls_params_t* p, uint64_t mode_seed, int32_t nc, int32_t nmodes,
+                                const double* gsl, const double* gu, const double* gv, const double* gw,
+                                double* g_coarse, double* g_amps, void* stream) {
+    int rc = flame_check(p, nc, nmodes);
+    if (rc) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const Coord c = make_coord(*p);
+    if (g_coarse) {
+        if (!gsl) return lsi::set_error(LS_ERR_INVALID, "gsl is NULL");
+        hipLaunchKernelGGL(k_flame_project_sl, dim3((unsigned)(nc * nc * nc)), dim3(256), 0, st, c, p->nx, p->ny,
+                           p->nz, make_coarse(*p, nc), gsl, g_coarse);
+        if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "projection launch failed");
+    }
+    if (g_amps && nmodes > 0) {
+        if (!gu || !gv || !gw) return lsi::set_error(LS_ERR_INVALID, "NULL velocity adjoint");
+        double* part = nullptr;
+        if (hipMalloc((void**)&part, kProjBlocks * kMaxModes * sizeof(double)) != hipSuccess)
+            return lsi::set_error(LS_ERR_NOMEM, "projection scratch allocation failed");
+        hipLaunchKernelGGL(k_flame_project_modes, dim3(kProjBlocks), dim3(256), 0, st, c, p->nx, p->ny, p->nz,
+                           draw_flame_modes(mode_seed, nmodes), gu, gv, gw, part);
+        hipLaunchKernelGGL(k_flame_project_final, dim3(1), dim3(64), 0, st, (int)nmodes, part, g_amps);
+        const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+        (void)hipFree(part);
+        if (!ok) return lsi::set_error(LS_ERR_DEVICE, "projection launch failed");
+    }
+    return LS_OK;
 }

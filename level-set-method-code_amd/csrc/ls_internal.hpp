@@ -10,6 +10,8 @@ namespace lsi {
 int set_error(int code, const char* msg);
 int validate(const ls_params_t* p);
 lsx::Geom make_geom(const ls_params_t& p);
+// The parameters a solver was created with.
+const ls_params_t& solver_params(const ls_solver_t* s);
 // Deformation velocity of the 3D sphere test at time t (ls_cases.hip).
 int launch_deformation_velocity(const ls_params_t& p, double t, double period, double* u,
                                 double* v, double* w, hipStream_t st);

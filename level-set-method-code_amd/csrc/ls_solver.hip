@@ -604,4 +604,5 @@ const char* ls_last_error(void) { return g_err.c_str(); }
 
 namespace lsi {
 int set_error(int code, const char* msg) { return fail(code, msg); }
+const ls_params_t& solver_params(const ls_solver_t* s) { return s->p; }
 }  // namespace lsi

@@ -266,3 +266,32 @@ class Reference(_Lib):
 
     def write_G(self, g, G, path):
         return self.lib.ref_write_G(C.byref(g), _p(G), str(path).encode())
+
+
+class Tangent:
+    """Forward-mode restatement of the 3D path (ls_tangent.cpp): the loss and
+    its derivative along one direction of (G0, u, v, w, S_L)."""
+
+    def __init__(self):
+        self.lib = C.CDLL(os.path.join(HERE, "liblstangent.so"))
+        self.lib.lst_run.restype = C.c_int
+
+    def run(self, g, G0, u, v, w, G_target, eps_h, dt, nsteps, reinit_interval, sl=None,
+            dG0=None, du=None, dv=None, dw=None, dsl=None):
+        """Returns (G_T, L, dL) with dL the directional derivative."""
+        out = np.zeros(g.shape)
+        L = C.c_double()
+        dL = C.c_double()
+        rc = self.lib.lst_run(C.byref(g), _p(G0), _p(dG0), _p(u), _p(du), _p(v), _p(dv), _p(w),
+                              _p(dw), _p(sl), _p(dsl), _p(G_target), C.c_double(eps_h),
+                              C.c_double(dt), C.c_int32(nsteps), C.c_int32(reinit_interval),
+                              _p(out), C.byref(L), C.byref(dL))
+        assert rc == 0, "lst_run rejected the grid"
+        return out, L.value, dL.value
+
+
+def heaviside(x, e):
+    """Smoothed Heaviside of the adjoint loss, numpy."""
+    x = np.asarray(x, np.float64)
+    h = 0.5 * (1.0 + x / e + np.sin(np.pi * x / e) / np.pi)
+    return np.where(x < -e, 0.0, np.where(x > e, 1.0, h))
