@@ -12,6 +12,8 @@ Prints ONE JSON line (rank 0).  Options:
   --mode exact|fast        arithmetic mode of the solver
   --n N                    grid edge (default 512)
   --no-cpu-baseline        skip the CPU reference timing
+  --no-adjoint             skip the reduced adjoint-vs-FD leg that the default
+                           line carries under "adjoint" (4 parameter sets)
   --profile-pass           launch a 1 GiB copy, a ghost fill and stage kernels
                            only (for rocprofv3 --pmc runs)
   --summarize-pmc F W      turn two rocprofv3 --pmc CSV directories (FETCH_SIZE,
@@ -205,7 +207,7 @@ def bench_batch(ls, steps, mode):
     print(json.dumps(out), flush=True)
 
 
-def bench_adjoint(ls, mode, nsets, fd_sample=16):
+def bench_adjoint(ls, mode, nsets, fd_sample=16, emit=True):
     """The adjoint configuration: parameter gradients of the flame-inversion
     loss for `nsets` seeded parameter sets vs central finite differences on
     the plain forward solver (2 forward solves per parameter per gradient)."""
@@ -267,8 +269,10 @@ def bench_adjoint(ls, mode, nsets, fd_sample=16):
            "fd_sample": {"parameters": len(fd), "h": h, "max_rel_err": err},
            "adjoint_workspace_bytes": prob.adj.workspace_bytes(),
            "losses": [g[0] for g in grads[:4]]}
-    print(json.dumps(out), flush=True)
     prob.close()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def read_traffic(n, mode):
@@ -359,6 +363,7 @@ def main():
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
     ap.add_argument("--config", choices=["c3", "batch", "adjoint"], default="c3")
     ap.add_argument("--sets", type=int, default=32)
+    ap.add_argument("--no-adjoint", action="store_true")
     a = ap.parse_args()
 
     if a.summarize_pmc:
@@ -425,6 +430,13 @@ def main():
            "roofline": roof}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.n)
+    if not a.no_adjoint:
+        # second half of the metric: adjoint vs finite-difference gradients (reduced batch)
+        adj = bench_adjoint(ls, a.mode, nsets=4, fd_sample=8, emit=False)
+        out["adjoint"] = {k: adj[k] for k in ("value", "unit", "adjoint_s_per_gradient", "forward_solve_s",
+                                               "fd_s_per_gradient", "speedup_vs_fd", "fd_sample")}
+        out["adjoint"]["workload"] = adj["config"]["workload"]
+        out["adjoint"]["parameter_sets"] = 4
     print(json.dumps(out), flush=True)
 
 
