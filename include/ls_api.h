@@ -305,6 +305,51 @@ int ls_flame_project(const ls_params_t* p, uint64_t mode_seed, int32_t nc, int32
                      const double* gsl, const double* gu, const double* gv, const double* gw,
                      double* g_coarse, double* g_amps, void* stream);
 
+/* ---- The sample/ scheme (2D, local Lax-Friedrichs) ------------------------
+ * The reference's second 2D solver (sample/): WENO-5 with its own smoothness
+ * indicators, an LLF Hamiltonian, RK3 in the forms 0.75 G + 0.25 (G1 + dt r)
+ * and 1/3 G + 2/3 (G2 + dt r), the anchored Bunsen boundary followed by the
+ * periodic fill after every stage.  Fields are padded (ny+6) x (nx+6)
+ * doubles, x fastest (Field2D::data[j][i] at j*(nx+6)+i).  Arithmetic equals
+ * the reference's CPU build bit for bit. */
+typedef struct ls_sample ls_sample_t;
+/* (nx+6)*(ny+6), or -1 when nx or ny < 6. */
+int64_t ls_sample_field_size(int32_t nx, int32_t ny);
+int ls_sample_create(int32_t nx, int32_t ny, double dx, double dy, ls_sample_t** out);
+int ls_sample_destroy(ls_sample_t* s);
+/* compute_geqn_rhs (sample/weno5_rk3.cpp:91-145): rhs on interior cells,
+ * zero elsewhere. */
+int ls_sample_rhs(ls_sample_t* s, const double* G, const double* u, const double* v, double S_L,
+                  double* rhs, void* stream);
+/* RK3_step (sample/weno5_rk3.cpp:148-188), G in place. */
+int ls_sample_rk3_step(ls_sample_t* s, double* G, const double* u, const double* v, double dt,
+                       double S_L, const double* G_initial, void* stream);
+/* Field2D::apply_periodic and apply_anchored_bc (sample/field.cpp:16-63). */
+int ls_sample_apply_periodic(ls_sample_t* s, double* G, void* stream);
+int ls_sample_apply_anchored_bc(ls_sample_t* s, double* G, const double* G_initial, void* stream);
+/* Field2D::reinitialize (sample/field.cpp:66-130). */
+int ls_sample_reinitialize(ls_sample_t* s, double* G, const double* G_initial, int32_t num_iters,
+                           void* stream);
+/* compute_first_order_upwind (sample/first_order_upwind.cpp:6-71), G in place. */
+int ls_sample_first_order_upwind(ls_sample_t* s, double* G, const double* u, const double* v,
+                                 double dt, double S_L, void* stream);
+/* The Bunsen-flame initial field of sample/main.cpp:95-116 (whole padded array). */
+int ls_sample_bunsen(ls_sample_t* s, double R, double flame_angle_deg, double X0, double Y0,
+                     double* G, void* stream);
+/* Uniform forced velocity u = u0 + eps_u v0 cos(2 pi freq t),
+ * v = v0 + eps_v v0 cos(2 pi freq t) (sample/main.cpp:131-136). */
+int ls_sample_forced_velocity(ls_sample_t* s, double t, int32_t freq, double u0, double v0,
+                              double eps_u, double eps_v, double* u, double* v, void* stream);
+/* nsteps iterations of the sample/main.cpp:126-148 loop body from time t0
+ * (boundary, forced velocity, RK3_step, boundary, t += dt); *t_out receives
+ * the final time.  Synchronises the stream. */
+int ls_sample_advance(ls_sample_t* s, double* G, double* u, double* v, const double* G_initial,
+                      double S_L, double dt, double t0, int64_t nsteps, int32_t freq, double u0,
+                      double v0, double eps_u, double eps_v, double* t_out, void* stream);
+/* save_G_to_bin (sample/main.cpp:34-51): interior rows of a HOST padded
+ * field, no header. */
+int ls_sample_write_G(int32_t nx, int32_t ny, const double* host_G, const char* path);
+
 /* Message for the last error on this thread (never NULL). */
 const char* ls_last_error(void);
 

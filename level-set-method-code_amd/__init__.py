@@ -128,6 +128,23 @@ def lib():
                                  C.POINTER(C.c_double), vp, vp, vp, vp, vp, vp]
     L.ls_flame_fields.argtypes = [P, C.c_uint64, C.c_int32, vp, C.c_int32, vp, vp, vp, vp, vp, vp]
     L.ls_flame_project.argtypes = [P, C.c_uint64, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
+    L.ls_sample_field_size.argtypes = [C.c_int32, C.c_int32]
+    L.ls_sample_field_size.restype = C.c_int64
+    L.ls_sample_create.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double, C.POINTER(vp)]
+    L.ls_sample_destroy.argtypes = [vp]
+    L.ls_sample_rhs.argtypes = [vp, vp, vp, vp, C.c_double, vp, vp]
+    L.ls_sample_rk3_step.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, vp, vp]
+    L.ls_sample_apply_periodic.argtypes = [vp, vp, vp]
+    L.ls_sample_apply_anchored_bc.argtypes = [vp, vp, vp, vp]
+    L.ls_sample_reinitialize.argtypes = [vp, vp, vp, C.c_int32, vp]
+    L.ls_sample_first_order_upwind.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, vp]
+    L.ls_sample_bunsen.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, vp, vp]
+    L.ls_sample_forced_velocity.argtypes = [vp, C.c_double, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, vp, vp, vp]
+    L.ls_sample_advance.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                    C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(C.c_double), vp]
+    L.ls_sample_write_G.argtypes = [C.c_int32, C.c_int32, vp, C.c_char_p]
     _lib = L
     return L
 
@@ -529,3 +546,98 @@ def flame_loss(sl_coarse, amps, problem):
     if _FLAME_LOSS is None:
         _FLAME_LOSS = _flame_loss_function()
     return _FLAME_LOSS.apply(sl_coarse, amps, problem)
+
+
+# ---------------------------------------------------------------------------
+# The sample/ scheme (2D, local Lax-Friedrichs)
+
+
+class Sample:
+    """Owns an ``ls_sample_t``: the reference's ``sample/`` solver on float64
+    device tensors of shape (ny+6, nx+6).  Defaults are sample/config.hpp's
+    (800 x 800 on a 0.02 m box, dt 2.5e-6 s, S_L 0.4, v0 0.8, 5% forcing)."""
+
+    def __init__(self, nx=800, ny=800, dx=None, dy=None, lx=0.02, ly=0.02):
+        self.nx, self.ny = nx, ny
+        self.dx = lx / (nx - 1) if dx is None else dx
+        self.dy = ly / (ny - 1) if dy is None else dy
+        self.shape = (ny + 6, nx + 6)
+        self.h = C.c_void_p()
+        check(lib().ls_sample_create(nx, ny, self.dx, self.dy, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ls_sample_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, *ts):
+        for t in ts:
+            if t is None:
+                continue
+            if not t.is_cuda or t.dtype.itemsize != 8 or not t.is_contiguous() or tuple(t.shape) != self.shape:
+                raise ValueError("fields must be contiguous float64 device tensors of shape %s" % (self.shape,))
+
+    def empty(self):
+        import torch
+        return torch.zeros(self.shape, dtype=torch.float64, device="cuda")
+
+    def rhs(self, G, u, v, S_L, stream=None):
+        self._chk(G, u, v)
+        r = self.empty()
+        check(lib().ls_sample_rhs(self.h, _ptr(G), _ptr(u), _ptr(v), S_L, _ptr(r), _stream(stream)))
+        return r
+
+    def rk3_step(self, G, u, v, dt, S_L, G_initial, stream=None):
+        self._chk(G, u, v, G_initial)
+        check(lib().ls_sample_rk3_step(self.h, _ptr(G), _ptr(u), _ptr(v), dt, S_L, _ptr(G_initial),
+                                       _stream(stream)))
+
+    def apply_periodic(self, G, stream=None):
+        self._chk(G)
+        check(lib().ls_sample_apply_periodic(self.h, _ptr(G), _stream(stream)))
+
+    def apply_anchored_bc(self, G, G_initial, stream=None):
+        self._chk(G, G_initial)
+        check(lib().ls_sample_apply_anchored_bc(self.h, _ptr(G), _ptr(G_initial), _stream(stream)))
+
+    def reinitialize(self, G, G_initial, num_iters, stream=None):
+        self._chk(G, G_initial)
+        check(lib().ls_sample_reinitialize(self.h, _ptr(G), _ptr(G_initial), num_iters, _stream(stream)))
+
+    def first_order_upwind(self, G, u, v, dt, S_L, stream=None):
+        self._chk(G, u, v)
+        check(lib().ls_sample_first_order_upwind(self.h, _ptr(G), _ptr(u), _ptr(v), dt, S_L,
+                                                 _stream(stream)))
+
+    def bunsen(self, R=0.005, angle=30.0, X0=0.01, Y0=0.0, stream=None):
+        G = self.empty()
+        check(lib().ls_sample_bunsen(self.h, R, angle, X0, Y0, _ptr(G), _stream(stream)))
+        return G
+
+    def forced_velocity(self, t, freq, u, v, u0=0.0, v0=0.8, eps_u=0.05, eps_v=0.05, stream=None):
+        self._chk(u, v)
+        check(lib().ls_sample_forced_velocity(self.h, t, freq, u0, v0, eps_u, eps_v, _ptr(u), _ptr(v),
+                                              _stream(stream)))
+
+    def advance(self, G, u, v, G_initial, nsteps, freq=10, t0=0.0, dt=2.5e-6, S_L=0.4, u0=0.0, v0=0.8,
+                eps_u=0.05, eps_v=0.05, stream=None):
+        """nsteps iterations of the sample driver loop; returns the final time."""
+        self._chk(G, u, v, G_initial)
+        t = C.c_double()
+        check(lib().ls_sample_advance(self.h, _ptr(G), _ptr(u), _ptr(v), _ptr(G_initial), S_L, dt, t0,
+                                      nsteps, freq, u0, v0, eps_u, eps_v, C.byref(t), _stream(stream)))
+        return t.value
+
+
+def sample_write_G(host_array, path):
+    """save_G_to_bin of a host (ny+6, nx+6) array."""
+    import numpy as np
+    a = np.ascontiguousarray(host_array, dtype=np.float64)
+    ny, nx = a.shape[0] - 6, a.shape[1] - 6
+    check(lib().ls_sample_write_G(nx, ny, C.c_void_p(a.ctypes.data), str(path).encode()))

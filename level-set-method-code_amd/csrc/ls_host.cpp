@@ -85,4 +85,19 @@ int ls_write_G(const ls_params_t* p, const double* host_G, const char* path) {
     return LS_OK;
 }
 
+// save_G_to_bin (sample/main.cpp:34-51): the interior rows only, no header.
+int ls_sample_write_G(int32_t nx, int32_t ny, const double* host_G, const char* path) {
+    if (nx < 6 || ny < 6) return lsi::set_error(LS_ERR_INVALID, "the sample scheme needs nx, ny >= 6");
+    if (!host_G || !path) return lsi::set_error(LS_ERR_INVALID, "NULL argument");
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return lsi::set_error(LS_ERR_IO, "cannot open output file");
+    const long long nxt = nx + 6;
+    bool ok = true;
+    for (long long j = 3; ok && j <= ny + 2; j++)
+        ok = std::fwrite(host_G + j * nxt + 3, sizeof(double), (size_t)nx, fp) == (size_t)nx;
+    if (std::fclose(fp) != 0) ok = false;
+    if (!ok) return lsi::set_error(LS_ERR_IO, "short write");
+    return LS_OK;
+}
+
 }  // extern "C"

@@ -295,3 +295,115 @@ def heaviside(x, e):
     x = np.asarray(x, np.float64)
     h = 0.5 * (1.0 + x / e + np.sin(np.pi * x / e) / np.pi)
     return np.where(x < -e, 0.0, np.where(x > e, 1.0, h))
+
+
+class _Sample:
+    """The sample/ scheme (2D, LLF Hamiltonian) on padded (ny+6, nx+6) arrays."""
+
+    def __init__(self, path, prefix):
+        self.lib = C.CDLL(path)
+        self.pre = prefix
+        for name in ("rhs", "rk3_step", "apply_periodic", "apply_anchored", "reinitialize",
+                     "first_order_upwind"):
+            getattr(self.lib, prefix + name).restype = None
+
+    def _f(self, name):
+        return getattr(self.lib, self.pre + name)
+
+    @staticmethod
+    def _n(a):
+        ny, nx = a.shape
+        return nx - 6, ny - 6
+
+    def rhs(self, G, u, v, dx, dy, S_L):
+        nx, ny = self._n(G)
+        r = np.zeros_like(G)
+        self._f("rhs")(nx, ny, _p(G), _p(u), _p(v), _p(r), C.c_double(dx), C.c_double(dy), C.c_double(S_L))
+        return r
+
+    def apply_periodic(self, G):
+        nx, ny = self._n(G)
+        G = G.copy()
+        self._f("apply_periodic")(nx, ny, _p(G))
+        return G
+
+    def apply_anchored(self, G, init):
+        nx, ny = self._n(G)
+        G = G.copy()
+        self._f("apply_anchored")(nx, ny, _p(G), _p(init))
+        return G
+
+
+class SampleOracle(_Sample):
+    """C restatement of sample/ (ls_sample.c, inside liblsoracle.so)."""
+
+    def __init__(self, fma=False):
+        super().__init__(os.path.join(HERE, "liblsoracle_fma.so" if fma else "liblsoracle.so"), "lss_")
+        self.lib.lss_write_G.restype = C.c_int
+        self.lib.lss_weno_left.restype = C.c_double
+        self.lib.lss_weno_right.restype = C.c_double
+        self.lib.lss_beta.restype = C.c_double
+
+    def rk3_step(self, G, u, v, dx, dy, dt, S_L, G_initial):
+        nx, ny = self._n(G)
+        G = G.copy()
+        work = np.zeros((3,) + G.shape)
+        self.lib.lss_rk3_step(nx, ny, _p(G), _p(u), _p(v), C.c_double(dx), C.c_double(dy), C.c_double(dt),
+                              C.c_double(S_L), _p(G_initial), _p(work))
+        return G
+
+    def reinitialize(self, G, init, dx, dy, iters):
+        nx, ny = self._n(G)
+        G = G.copy()
+        work = np.zeros(G.shape)
+        self.lib.lss_reinitialize(nx, ny, _p(G), _p(init), C.c_double(dx), C.c_double(dy), iters, _p(work))
+        return G
+
+    def first_order_upwind(self, G, u, v, dx, dy, dt, S_L):
+        nx, ny = self._n(G)
+        G = G.copy()
+        work = np.zeros(G.shape)
+        self.lib.lss_first_order_upwind(nx, ny, _p(G), _p(u), _p(v), C.c_double(dx), C.c_double(dy),
+                                        C.c_double(dt), C.c_double(S_L), _p(work))
+        return G
+
+    def bunsen(self, n, dx, dy, R=0.005, angle=30.0, X0=0.01, Y0=0.0):
+        G = np.zeros((n + 6, n + 6))
+        self.lib.lss_bunsen(n, n, C.c_double(dx), C.c_double(dy), C.c_double(R), C.c_double(angle),
+                            C.c_double(X0), C.c_double(Y0), _p(G))
+        return G
+
+    def write_G(self, G, path):
+        nx, ny = self._n(G)
+        return self.lib.lss_write_G(nx, ny, _p(G), str(path).encode())
+
+
+def sample_ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libsample.so"))
+
+
+class SampleReference(_Sample):
+    """The reference's own sample/ functions compiled from its sources (oracle/_ref)."""
+
+    def __init__(self):
+        super().__init__(os.path.join(HERE, "_ref", "libsample.so"), "refs_")
+
+    def rk3_step(self, G, u, v, dx, dy, dt, S_L, G_initial):
+        nx, ny = self._n(G)
+        G = G.copy()
+        self.lib.refs_rk3_step(nx, ny, _p(G), _p(u), _p(v), C.c_double(dx), C.c_double(dy), C.c_double(dt),
+                               C.c_double(S_L), _p(G_initial))
+        return G
+
+    def reinitialize(self, G, init, dx, dy, iters):
+        nx, ny = self._n(G)
+        G = G.copy()
+        self.lib.refs_reinitialize(nx, ny, _p(G), _p(init), C.c_double(dx), C.c_double(dy), iters)
+        return G
+
+    def first_order_upwind(self, G, u, v, dx, dy, dt, S_L):
+        nx, ny = self._n(G)
+        G = G.copy()
+        self.lib.refs_first_order_upwind(nx, ny, _p(G), _p(u), _p(v), C.c_double(dx), C.c_double(dy),
+                                         C.c_double(dt), C.c_double(S_L))
+        return G
