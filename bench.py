@@ -23,6 +23,9 @@ Prints ONE JSON line (rank 0).  Options:
                            S_L ~ U[0.05,0.2], 300 RK3 steps each, in one
                            autotuned, hipGraph-replayed batch vs a serial
                            per-instance loop (prints its own JSON line)
+  --config sample          the reference's sample/ scheme instead: its 800x800
+                           Bunsen-flame driver loop, 2000 steps, next to the
+                           reference's own sample code on 16 CPU threads
   --config adjoint         the adjoint configuration instead: flame-inversion
                            gradients (128^3, 8^3 S_L nodes + 16 mode amplitudes,
                            100 steps) for --sets seeded parameter sets vs
@@ -275,6 +278,60 @@ def bench_adjoint(ls, mode, nsets, fd_sample=16, emit=True):
     return out
 
 
+def bench_sample(ls, steps, n=800):
+    """The sample/ scheme: the reference's Bunsen-flame driver loop (800 x 800,
+    dt 2.5e-6, S_L 0.4, forced velocity at 10 Hz) on the GPU, with the
+    reference's own sample code (OpenMP, 16 threads) timed beside it."""
+    import torch
+    s = ls.Sample(n, n)
+    G0 = s.bunsen()
+    G = G0.clone()
+    u, v = s.empty(), s.empty()
+    t = s.advance(G, u, v, G0, 20, freq=10)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t = s.advance(G, u, v, G0, steps, freq=10, t0=t)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # RK3 kernels alone (3 stage launches), HIP events on the launch stream
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    e0.record(st)
+    for _ in range(reps):
+        s.rk3_step(G, u, v, 2.5e-6, 0.4, G0)
+    e1.record(st)
+    torch.cuda.synchronize()
+    t_step = e0.elapsed_time(e1) * 1e-3 / reps
+    cells = n * n
+    bytes_step = 112  # stage 1: G,u,v -> G1 32 B; stages 2, 3: 40 B each (algorithmic, per cell)
+    ach = bytes_step * cells / t_step
+    out = {"metric": "sample/ scheme cell-updates/s (LLF WENO5-RK3, 800² Bunsen flame, FP64)",
+           "value": cells * steps / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "sample/main.cpp Bunsen flame %dx%d on 0.02 m, dt 2.5e-6, S_L 0.4, "
+                                  "forced velocity 10 Hz, anchored+periodic boundary" % (n, n), "grid": [n, n]},
+           "ms_per_step": wall / steps * 1e3,
+           "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK, "traffic": None,
+                        "kernel": "k_sample_stage<1..3> (3 launches per RK3 step)",
+                        "rk3_kernels_ms": t_step * 1e3, "algorithmic_bytes_per_cell_step": bytes_step}}
+    exe = os.path.join(ROOT, "oracle", "_ref", "sample_bench")
+    if os.path.exists(exe):
+        env = dict(os.environ, OMP_NUM_THREADS="16")
+        cpu_steps = 40
+        r = subprocess.run([exe, str(n), str(cpu_steps)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, env=env, timeout=600)
+        if r.returncode == 0:
+            j = json.loads(r.stdout.strip().splitlines()[-1])
+            out["cpu_baseline"] = {"value": j["cell_updates_per_s"], "unit": "cell-updates/s",
+                                   "cores": j["threads"], "kind": "reference",
+                                   "sample": "the reference's sample/ RK3_step loop (its own sources and flags, "
+                                             "OpenMP) on the same %dx%d case, %d steps" % (n, n, cpu_steps)}
+    print(json.dumps(out), flush=True)
+    s.close()
+
+
 def read_traffic(n, mode):
     if n != 512 or not os.path.exists(TRAFFIC_FILE):
         return None
@@ -361,7 +418,7 @@ def main():
     ap.add_argument("--profile-pass", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
-    ap.add_argument("--config", choices=["c3", "batch", "adjoint"], default="c3")
+    ap.add_argument("--config", choices=["c3", "batch", "adjoint", "sample"], default="c3")
     ap.add_argument("--sets", type=int, default=32)
     ap.add_argument("--no-adjoint", action="store_true")
     a = ap.parse_args()
@@ -386,6 +443,9 @@ def main():
         return
     if a.config == "adjoint":
         bench_adjoint(ls, a.mode, a.sets)
+        return
+    if a.config == "sample":
+        bench_sample(ls, 2000 if a.steps == 20 else a.steps)
         return
 
     p, s, G, u, v, w, dt = setup(ls, a.n, a.mode)

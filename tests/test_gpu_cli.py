@@ -61,3 +61,40 @@ def test_cli_sphere_reinit_matches_oracle(cuda, tmp_path, mode, tol):
     I = g.interior
     # device sin/cos may differ from glibc in the last bit: 1e-12, not bitwise
     assert rel_linf(got[I], ref[I]) <= tol
+
+
+SAMPLE_EXE = os.path.join(ROOT, "level-set-method-code_amd", "ls_sample_solve")
+
+
+def test_sample_cli_matches_oracle_loop(cuda, tmp_path):
+    """ls_sample_solve (the sample/main.cpp driver) writes the same G files as
+    the restated loop: step count from the accumulated-time loop condition,
+    save files every 10 steps, interior rows only."""
+    import math
+    from oracle.pyoracle import SampleOracle
+    n, dt, T, save = 48, 2.5e-6, 25 * 2.5e-6, 10
+    r = subprocess.run([SAMPLE_EXE, "10", "-n", str(n), "-T", repr(T), "-save", str(save), "-o",
+                        str(tmp_path / "res")], cwd=tmp_path, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    O = SampleOracle()
+    dx = 0.02 / (n - 1)
+    G0 = O.bunsen(n, dx, dx)
+    G = G0.copy()
+    u, v = np.empty_like(G), np.empty_like(G)
+    t, step, saved = 0.0, 0, {0: G0.copy()}
+    while t < T:
+        G = O.apply_periodic(O.apply_anchored(G, G0))
+        u.fill(0.0 + 0.05 * 0.8 * math.cos(10 * 2 * math.pi * (t)))
+        v.fill(0.8 + 0.05 * 0.8 * math.cos(10 * 2 * math.pi * (t)))
+        G = O.rk3_step(G, u, v, dx, dx, dt, 0.4, G0)
+        G = O.apply_anchored(O.apply_periodic(G), G0)
+        t += dt
+        step += 1
+        if step % save == 0:
+            saved[step // save] = G.copy()
+    assert "Simulation finished" in r.stdout
+    for k, Gk in saved.items():
+        got = np.fromfile(tmp_path / "res" / "10Hz" / ("itr_G_field_%d.bin" % k), dtype=np.float64)
+        assert np.array_equal(got, Gk[3:-3, 3:-3].ravel()), k
+    assert "%d steps" % step in r.stdout
