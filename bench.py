@@ -49,6 +49,8 @@ HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
 STAGE_BYTES = {1: 40, 2: 48, 3: 48}   # algorithmic bytes per interior cell per stage
 STEP_BYTES = 136 + 83 / 5.0           # RK3 + amortised HCR-2, bytes per cell-update
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_traffic_512.json")
+# the 3D fast-mode stage kernel (shared-face z-march)
+FAST_STAGE_KERNEL = "k_stage_zface<2>"
 
 
 def load_ls():
@@ -383,7 +385,7 @@ def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=TRAFFIC_FILE):
     fr = per_kernel(_pmc_rows(fetch_dir), "FETCH_SIZE")
     wr = per_kernel(_pmc_rows(write_dir), "WRITE_SIZE")
     copy_k = [k for k in fr if "MulFunctor" in k]
-    stage_k = [k for k in fr if ((mode == "fast" and "k_stage_zmarch<2>" in k) or
+    stage_k = [k for k in fr if ((mode == "fast" and FAST_STAGE_KERNEL in k) or
                                  (mode == "exact" and "k_stage_exact<3, 2>" in k))]
     if not copy_k or not stage_k:
         raise SystemExit("kernels not found in pmc output: %s" % list(fr))
@@ -474,7 +476,7 @@ def main():
     traffic = read_traffic(a.n, a.mode)
     roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": ach / HBM_PEAK, "traffic": traffic,
-            "kernel": ("k_stage_zmarch<2>" if a.mode == "fast" else "k_stage_exact<3,2>") +
+            "kernel": (FAST_STAGE_KERNEL if a.mode == "fast" else "k_stage_exact<3,2>") +
                       " (fused WENO5/u_eff RHS + stage-2 combine)",
             "kernel_ms": t_stage * 1e3, "ghost_fill_ms": t_fill * 1e3,
             "algorithmic_bytes_per_cell": STAGE_BYTES[2],

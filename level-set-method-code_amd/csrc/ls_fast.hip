@@ -15,7 +15,11 @@
 //    which equals d_k/t_k / sum_j d_j/t_j without any division;
 //  * the two face values enter the derivative as one fraction
 //        (N_b D_a - N_a D_b) / (6 h D_a D_b),
-//    so each axis costs a single reciprocal (v_rcp_f64 + Newton steps);
+//    so each axis costs a single reciprocal (v_rcp_f64 + Newton steps); the
+//    3D stage kernel goes further and puts all three axes over one
+//    denominator, one reciprocal per cell;
+//  * the 3D stage kernel reconstructs each face once, in the orientations
+//    its two cells need, and shares it between them (k_stage_zface);
 //  * constant divisions (by 6, 2h, h) become multiplies, FMA is used freely.
 #include <hip/hip_runtime.h>
 
@@ -180,22 +184,17 @@ __global__ __launch_bounds__(256) void k_hcr2_fast(FastGeom f, const double* __r
 
 
 // ---------------------------------------------------------------------------
-// 3D stage kernel, z-marching.  A block owns a 64 x 4 column of cells and
-// walks a chunk of z planes.  Each thread keeps its column's 7 G values along
-// z and the 5 z smoothness terms in registers, so the z stencil costs one
-// load and one tsq per plane.  The x/y stencils come from an LDS copy of the
-// current plane (tile + 3-cell halo); x and y smoothness terms are computed
-// once per point into LDS and shared by the neighbours that need them.
-// Chunks are assigned so that each XCD walks one z-slab: the y-halo rows a
-// block reads were loaded as its neighbours' leading planes moments before
-// and are still in that XCD's L2.  Ghost cells of `out` are written by the
-// interior cell they copy (periodic / zero-gradient images), which replaces
-// the separate ghost-fill launch.
+// 3D kernels march along z: a block owns a column of cells of the x-y plane
+// and walks a chunk of z planes, keeping the column's z stencil in registers
+// and the current plane (tile + halo) in LDS.  Chunks are assigned so that
+// each XCD walks one z-slab: the y-halo rows a block reads were loaded as its
+// neighbours' planes moments before and are still in that XCD's L2.  Ghost
+// cells of the output are written by the interior cell they copy (periodic /
+// zero-gradient images), which replaces the separate ghost-fill launch.
 constexpr int ZX = 64, ZY = 4, ZH = 3;
-constexpr int SGW = ZX + 2 * ZH, SGH = ZY + 2 * ZH;
 
-// Kernel arguments of the z-marching stage kernel, kept small so that they
-// stay in scalar registers.
+// Kernel arguments of the z-marching kernels, kept small so that they stay in
+// scalar registers.
 struct ZGeom {
     int32_t nx, ny, nz;
     int32_t sy, sz;               // strides in doubles (sz*nzt < 2^31 checked at launch)
@@ -219,196 +218,291 @@ __device__ __forceinline__ int image(int q, int n, int bc, int a) {
     return q == 0 ? a - 1 : n + 2 + a;  // zero-gradient: 0,1,2 or n+3,n+4,n+5
 }
 
-// Loads for a later plane: the leading G of the column, this thread's halo
-// cells and the pointwise u, v, w, G_n (and S_L field) are issued one or two
-// planes ahead, so their latency overlaps the arithmetic of the current plane.
-#ifndef LS_ZDBUF
-#define LS_ZDBUF 0
-#endif
-#ifndef LS_ZPF2
-#define LS_ZPF2 0
-#endif
-struct ZHalo {
-    double own, h0, h1, xh;
-};
-struct ZPoint {
-    double lead, u, v, w, gn, sl;
+// ---------------------------------------------------------------------------
+// 3D stage kernel with shared faces.  The upwind derivative of a cell along
+// an axis is (F(c+1/2) - F(c-1/2)) / h, where both faces are reconstructed
+// in the cell's upwind orientation: left-biased (weno5_left on G(c-2..c+2)
+// for the face c+1/2) when u_eff >= 0, mirrored (weno5_right) otherwise
+// (Local_code/level-set_MPI_3D/include/weno5.h:90-196).  A face is shared by
+// the two cells beside it, so it is reconstructed once, in each orientation
+// one of those cells needs -- usually both cells agree and one reconstruction
+// serves two cells:
+//  * x and y faces: after the cells of a plane have their u_eff, each cell
+//    builds the face on its high side in the orientations of itself and its
+//    high neighbour and leaves it in LDS; the faces on the low edge of the
+//    tile are built by spare lanes of two waves;
+//  * z faces: a column builds the face k+1/2 in the orientation of cell k
+//    and carries it to plane k+1, which rebuilds it only where its own
+//    orientation differs.
+// A face is kept as numerator n and denominator d, F = n / (6 d), with the
+// product-form weights (see face_nd).  The three axis derivatives enter one
+// fraction, so a cell costs a single refined reciprocal:
+//    rhs = -(sum_a (u_a / 6h_a) N_a prod_{b != a} D_b) / prod_a D_a,
+//    N_a = n_hi d_lo - n_lo d_hi,  D_a = d_hi d_lo.
+// Smoothness terms T(c) = (eps + beta)^2 of the sub-stencil centred at c do
+// not depend on the orientation and are computed once per point.
+struct Face {
+    double n, d;
 };
 
-__device__ __forceinline__ void z_halo(const ZGeom& z, int k, bool active, int i, int j, int x0, int y0, int tid,
-                                       const double* __restrict__ Gs, ZHalo& n) {
+// Face between cells c and c+1.  v0..v4 run from the far upwind side: G(c-2..c+2)
+// left-biased, G(c+3..c-1) mirrored; t0..t2 are T at the centres of v1..v3.
+// The linear weights (0.1, 0.6, 0.3) / (0.3, 0.6, 0.1) are scaled by 10,
+// a factor common to numerator and denominator.
+template <bool MIRROR>
+__device__ __forceinline__ Face face_core(double v0, double v1, double v2, double v3, double v4, double t0,
+                                          double t1, double t2) {
+    const double p0 = fma(2.0, v0, fma(-7.0, v1, 11.0 * v2));
+    const double p1 = fma(5.0, v2, fma(2.0, v3, -v1));
+    const double p2 = fma(2.0, v2, fma(5.0, v3, -v4));
+    const double w0 = MIRROR ? 3.0 * (t1 * t2) : t1 * t2;
+    const double w1 = 6.0 * (t0 * t2);
+    const double w2 = MIRROR ? t0 * t1 : 3.0 * (t0 * t1);
+    Face f;
+    f.n = fma(w0, p0, fma(w1, p1, w2 * p2));
+    f.d = (w0 + w1) + w2;
+    return f;
+}
+
+// The face c+1/2 from LDS: g points at G(c), t at T(c); GS and TS are the
+// strides of the axis in g and t.
+template <bool MIRROR, int GS, int TS>
+__device__ __forceinline__ Face face_lds(const double* g, const double* t) {
+    if (MIRROR) return face_core<true>(g[3 * GS], g[2 * GS], g[GS], g[0], g[-GS], t[2 * TS], t[TS], t[0]);
+    return face_core<false>(g[-2 * GS], g[-GS], g[0], g[GS], g[2 * GS], t[-TS], t[0], t[TS]);
+}
+
+// Stores the face c+1/2 in each orientation asked for: slot[0] left-biased,
+// slot[OS] mirrored.
+template <int GS, int TS, int OS>
+__device__ __forceinline__ void face_put(const double* g, const double* t, bool needL, bool needR, double2* slot) {
+    if (needL) {
+        const Face f = face_lds<false, GS, TS>(g, t);
+        slot[0] = make_double2(f.n, f.d);
+    }
+    if (needR) {
+        const Face f = face_lds<true, GS, TS>(g, t);
+        slot[OS] = make_double2(f.n, f.d);
+    }
+}
+
+// Ghost images of a freshly computed interior value (periodic / zero-gradient).
+__device__ __forceinline__ void store_images(const ZGeom& z, double* __restrict__ out, int i, int j, int k,
+                                             double o) {
     const int64_t sy = z.sy, sz = z.sz;
-    const int64_t plane = (int64_t)(k + 3) * sz;
-    n.own = (!active && i < z.nx + 3 && j < z.ny + 3) ? Gs[plane + (int64_t)(j + 3) * sy + (i + 3)] : 0.0;
-    {
-        const int h = tid;  // y halo cell (rows 0-2 and ZY+3..ZY+5); tid < 6*ZX always
-        const int hr = h / ZX, hx = h - hr * ZX;
-        const int row = hr < 3 ? hr : hr + ZY;
-        const int gy = y0 - 3 + row, gx = x0 + hx;
-        n.h0 = (gx < z.nx && gy < z.ny + 3) ? Gs[plane + (int64_t)(gy + 3) * sy + (gx + 3)] : 0.0;
-    }
-    n.h1 = 0.0;
-    if (tid + ZX * ZY < 6 * ZX) {
-        const int h = tid + ZX * ZY;
-        const int hr = h / ZX, hx = h - hr * ZX;
-        const int row = hr < 3 ? hr : hr + ZY;
-        const int gy = y0 - 3 + row, gx = x0 + hx;
-        n.h1 = (gx < z.nx && gy < z.ny + 3) ? Gs[plane + (int64_t)(gy + 3) * sy + (gx + 3)] : 0.0;
-    }
-    n.xh = 0.0;
-    if (tid < 6 * ZY) {
-        const int c6 = tid % 6, rr = tid / 6;
-        const int cx = c6 < 3 ? c6 : c6 + ZX;
-        const int gx = x0 - 3 + cx, gy = y0 + rr;
-        n.xh = (gx < z.nx + 3 && gy < z.ny) ? Gs[plane + (int64_t)(gy + 3) * sy + (gx + 3)] : 0.0;
+    const int nxi = n_images(i, z.nx, z.bc[0]);
+    const int nyi = n_images(j, z.ny, z.bc[1]);
+    const int nzi = n_images(k, z.nz, z.bc[2]);
+    for (int a = 0; a < nzi; a++) {
+        const int64_t pk = (int64_t)image(k, z.nz, z.bc[2], a) * sz;
+        for (int b = 0; b < nyi; b++) {
+            const int64_t pj = pk + (int64_t)image(j, z.ny, z.bc[1], b) * sy;
+            for (int e = 0; e < nxi; e++)
+                if (a | b | e) out[pj + image(i, z.nx, z.bc[0], e)] = o;
+        }
     }
 }
 
-__device__ __forceinline__ void z_point(const ZGeom& z, int k, bool active, int64_t col,
-                                        const double* __restrict__ Gs, const double* __restrict__ Gn,
-                                        const double* __restrict__ u, const double* __restrict__ v,
-                                        const double* __restrict__ w, const double* __restrict__ sl,
-                                        ZPoint& n) {
-    const int64_t plane = (int64_t)(k + 3) * z.sz;
-    if (active) {
-        const int64_t c = plane + col;
-        n.lead = Gs[c + 3 * (int64_t)z.sz];
-        n.u = u[c];
-        n.v = v[c];
-        n.w = w[c];
-        n.gn = Gn[c];
-        n.sl = sl ? sl[c] : z.s_l;
-    } else {
-        n.lead = n.u = n.v = n.w = n.gn = n.sl = 0.0;
-    }
-}
+// Tile of the shared-face kernel: 64 x 4 cells per plane, one cell per
+// thread, one wave per row.  Besides its row, each wave takes a share of the
+// tile's edge work: every wave one row of the y-edge smoothness terms, wave 2
+// the x-edge terms, wave 0 the x faces and wave 1 the y faces on the low
+// tile edges.
+constexpr int FX = 64, FY = 4;
+constexpr int FGH = FY + 2 * ZH, FGW = FX + 2 * ZH;
+constexpr int kFaceThreads = FX * FY;
+constexpr int kHaloItems = 6 * FX + 6 * FY;   // G halo values per plane
+constexpr int NH = (kHaloItems + kFaceThreads - 1) / kFaceThreads;  // per thread
 
-#ifndef LS_ZMARCH_WAVES
-#define LS_ZMARCH_WAVES 3
-#endif
 template <int STAGE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LS_ZMARCH_WAVES, 8))) void k_stage_zmarch(
+__global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_stage_zface(
     ZGeom z, const double* __restrict__ Gs, const double* __restrict__ Gn, double* __restrict__ out,
     const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
     const double* __restrict__ sl, double dt) {
-    constexpr int NBUF = LS_ZDBUF ? 2 : 1;
-    __shared__ double sG[NBUF][SGH][SGW];
-    __shared__ double sTx[NBUF][ZY][ZX + 4];
-    __shared__ double sTy[NBUF][ZY + 4][ZX];
+    __shared__ double sG[FGH][FGW];            // plane k: tile + 3-cell halo
+    __shared__ double sTx[FY][FX + 4];         // T along x at local x = -2..FX+1
+    __shared__ double sTy[FY + 4][FX];         // T along y at local y = -2..FY+1
+    __shared__ double2 sFx[2][FY][FX + 1];     // [left-biased, mirrored] x face between local x s-1 and s
+    __shared__ double2 sFy[2][FY + 1][FX];     // [left-biased, mirrored] y face between local y s-1 and s
+    __shared__ unsigned char sOri[FY][FX];     // bit 0: u_eff >= 0, bit 1: v_eff >= 0
     const int64_t t = xcd_tile(blockIdx.x, z.nblocks);
     if (t >= z.ntiles) return;  // block-uniform
     const int tx = (int)(t % z.tiles_x);
     const int64_t r = t / z.tiles_x;
     const int ty = (int)(r % z.tiles_y);
     const int ch = (int)(r / z.tiles_y);
-    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * ZX + lx;
-    const int x0 = tx * ZX, y0 = ty * ZY;
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * FX + lx;
+    const int x0 = tx * FX, y0 = ty * FY;
     const int i = x0 + lx, j = y0 + ly;
     const bool active = i < z.nx && j < z.ny;
     const int kb = ch * z.kz;
     const int ke = min(kb + z.kz, z.nz);
-    const int64_t sy = z.sy, sz = z.sz;
-    const int64_t col = (int64_t)(j + 3) * sy + (i + 3);
+    const int64_t sz = z.sz;
+    const double c2 = 0.25 * dt, c3 = (2.0 / 3.0) * dt;
 
+    // Loads are unconditional: a position past the padded array reads the
+    // nearest padded cell instead.  Such positions feed only the stencils of
+    // cells outside the interior.  Offsets within a plane are int32 (the
+    // launcher checks the padded size).
+    auto pad = [&](int gx, int gy) { return (min(gy, z.ny + 2) + 3) * z.sy + min(gx, z.nx + 2) + 3; };
+    const int32_t col = pad(i, j);
+    // Halo items tid + a * kFaceThreads: below 6 FX the rows at local y
+    // -3..-1 and FY..FY+2, then 6 FY items the columns at local x -3..-1 and
+    // FX..FX+2.
+    int hrow[NH], hcol[NH];
+    int32_t ohalo[NH];
+#pragma unroll
+    for (int a = 0; a < NH; a++) {
+        const int h = min(tid + a * kFaceThreads, kHaloItems - 1);
+        if (h < 6 * FX) {
+            const int q = h / FX;
+            hrow[a] = q < 3 ? q : q + FY;
+            hcol[a] = h - q * FX + 3;
+        } else {
+            const int e = h - 6 * FX, c6 = e % 6;
+            hrow[a] = e / 6 + 3;
+            hcol[a] = c6 < 3 ? c6 : c6 + FX;
+        }
+        ohalo[a] = pad(x0 - 3 + hcol[a], y0 - 3 + hrow[a]);
+    }
+
+    struct Next {
+        double lead, u, v, w, gn, sl, halo[NH];
+    };
+    auto fetch = [&](int kk, Next& n) {  // plane kk's point data and halo values, and G(kk+3) of the column
+        const int64_t pl = (int64_t)(kk + 3) * sz;
+        n.lead = Gs[pl + 3 * sz + col];
+        n.u = u[pl + col];
+        n.v = v[pl + col];
+        n.w = w[pl + col];
+        n.gn = Gn[pl + col];
+        n.sl = sl ? sl[pl + col] : z.s_l;
+#pragma unroll
+        for (int a = 0; a < NH; a++) n.halo[a] = Gs[pl + ohalo[a]];
+    };
+    auto put_plane = [&](double own, const Next& n) {
+        sG[ly + 3][lx + 3] = own;
+#pragma unroll
+        for (int a = 0; a < NH; a++)
+            if (tid + a * kFaceThreads < kHaloItems) sG[hrow[a]][hcol[a]] = n.halo[a];
+    };
+
+    // Column window along z: zG[q] = G(k-3+q), zt[q] = T(k-2+q).
     double zG[7], zt[5];
 #pragma unroll
-    for (int q = 0; q < 6; q++) zG[q] = active ? Gs[(int64_t)(kb + q) * sz + col] : 0.0;
+    for (int q = 0; q < 6; q++) zG[q] = Gs[(int64_t)(kb + q) * sz + col];
     zG[6] = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; q++) zt[q] = tsq(zG[q], zG[q + 1], zG[q + 2]);
     zt[4] = 0.0;
-    ZHalo nh;
-    ZPoint np1, np2;
-    z_halo(z, kb, active, i, j, x0, y0, tid, Gs, nh);
-    z_point(z, kb, active, col, Gs, Gn, u, v, w, sl, np1);
-    if (LS_ZPF2 && kb + 1 < ke) z_point(z, kb + 1, active, col, Gs, Gn, u, v, w, sl, np2);
+    Next nx;
+    fetch(kb, nx);
+    put_plane(zG[3], nx);
+    Face zlo;          // z face k-1/2, built in orientation zlo_L
+    zlo.n = zlo.d = 0.0;
+    int zlo_L = -1;    // none yet
 
     for (int k = kb; k < ke; k++) {
-        const int bf = LS_ZDBUF ? (k & 1) : 0;
         const int64_t plane = (int64_t)(k + 3) * sz;
-        zG[6] = np1.lead;
+        zG[6] = nx.lead;
         zt[4] = tsq(zG[4], zG[5], zG[6]);
-        const double cu = np1.u, cv = np1.v, cw = np1.w, cgn = np1.gn, csl = np1.sl;
-        if (LS_ZPF2) {
-            np1 = np2;
-            if (k + 2 < ke) z_point(z, k + 2, active, col, Gs, Gn, u, v, w, sl, np2);
-        } else if (k + 1 < ke) {
-            z_point(z, k + 1, active, col, Gs, Gn, u, v, w, sl, np1);
-        }
-        if (!LS_ZDBUF) __syncthreads();  // single buffer: previous plane's reads done
-        // Threads past the interior of a partial tile still supply their cell
-        // (a ghost value within the padded array) to the stencils of the
-        // interior cells next to them.
-        sG[bf][ly + 3][lx + 3] = active ? zG[3] : nh.own;
+        const double cu = nx.u, cv = nx.v, cw = nx.w, cgn = nx.gn, csl = nx.sl;
+        if (k + 1 < ke) fetch(k + 1, nx);
+        __syncthreads();  // plane k in sG
+        // --- smoothness terms along x and y, effective velocity, orientation
+        sTx[ly][lx + 2] = tsq(sG[ly + 3][lx + 2], sG[ly + 3][lx + 3], sG[ly + 3][lx + 4]);
+        sTy[ly + 2][lx] = tsq(sG[ly + 2][lx + 3], sG[ly + 3][lx + 3], sG[ly + 4][lx + 3]);
         {
-            const int hr = tid / ZX;
-            sG[bf][hr < 3 ? hr : hr + ZY][tid - hr * ZX + 3] = nh.h0;
+            const int ry = ly < 2 ? ly : ly + FY;  // local rows -2, -1, FY, FY+1
+            sTy[ry][lx] = tsq(sG[ry][lx + 3], sG[ry + 1][lx + 3], sG[ry + 2][lx + 3]);
         }
-        if (tid + ZX * ZY < 6 * ZX) {
-            const int h = tid + ZX * ZY, hr = h / ZX;
-            sG[bf][hr < 3 ? hr : hr + ZY][h - hr * ZX + 3] = nh.h1;
+        if (ly == 2 && lx < 4 * FY) {  // local x -2, -1, FX, FX+1 of each row
+            const int row = lx >> 2, c4 = lx & 3;
+            const int q = c4 < 2 ? c4 : c4 + FX;
+            sTx[row][q] = tsq(sG[row + 3][q], sG[row + 3][q + 1], sG[row + 3][q + 2]);
         }
-        if (tid < 6 * ZY) {
-            const int c6 = tid % 6;
-            sG[bf][tid / 6 + 3][c6 < 3 ? c6 : c6 + ZX] = nh.xh;
+        double ue = cu, ve = cv, we = cw;
+        if (csl > z.eps) {
+            const double gx = (sG[ly + 3][lx + 4] - sG[ly + 3][lx + 2]) * z.inv2dx;
+            const double gy = (sG[ly + 4][lx + 3] - sG[ly + 2][lx + 3]) * z.inv2dy;
+            const double gz = (zG[4] - zG[2]) * z.inv2dz;
+            const double s2 = fma(gz, gz, fma(gx, gx, gy * gy));
+            const double kk = csl * rsqrt_nr(s2 + z.eps);
+            ue = fma(-kk, gx, ue);
+            ve = fma(-kk, gy, ve);
+            we = fma(-kk, gz, we);
         }
-        if (k + 1 < ke) z_halo(z, k + 1, active, i, j, x0, y0, tid, Gs, nh);
-        __syncthreads();
-        sTx[bf][ly][lx + 2] = tsq(sG[bf][ly + 3][lx + 2], sG[bf][ly + 3][lx + 3], sG[bf][ly + 3][lx + 4]);
-        if (lx < 4) {
-            const int q = lx < 2 ? lx : lx + ZX;
-            sTx[bf][ly][q] = tsq(sG[bf][ly + 3][q], sG[bf][ly + 3][q + 1], sG[bf][ly + 3][q + 2]);
-        }
-        sTy[bf][ly + 2][lx] = tsq(sG[bf][ly + 2][lx + 3], sG[bf][ly + 3][lx + 3], sG[bf][ly + 4][lx + 3]);
-        {
-            const int q = ly < 2 ? ly : ly + ZY;
-            sTy[bf][q][lx] = tsq(sG[bf][q][lx + 3], sG[bf][q + 1][lx + 3], sG[bf][q + 2][lx + 3]);
-        }
-        __syncthreads();
+        const bool Lx = ue >= 0.0, Ly = ve >= 0.0, Lz = we >= 0.0;
+        sOri[ly][lx] = (unsigned char)((Lx ? 1 : 0) | (Ly ? 2 : 0));
+        __syncthreads();  // T, orientations
+        // --- faces
+        Face zhi;
+        zhi.n = zhi.d = 0.0;
         if (active) {
-            const int64_t c = plane + col;
-            double ue = cu, ve = cv, we = cw;
-            const double gc = zG[3];
-            if (csl > z.eps) {
-                const double gx = (sG[bf][ly + 3][lx + 4] - sG[bf][ly + 3][lx + 2]) * z.inv2dx;
-                const double gy = (sG[bf][ly + 4][lx + 3] - sG[bf][ly + 2][lx + 3]) * z.inv2dy;
-                const double gz = (zG[4] - zG[2]) * z.inv2dz;
-                const double s2 = fma(gz, gz, fma(gx, gx, gy * gy));
-                const double kk = csl * rsqrt_nr(s2 + z.eps);
-                ue = fma(-kk, gx, ue);
-                ve = fma(-kk, gy, ve);
-                we = fma(-kk, gz, we);
+            const double* g = &sG[ly + 3][lx + 3];
+            {  // x face i+1/2, for this cell and its high neighbour
+                bool nL = Lx, nR = !Lx;
+                if (lx + 1 < FX && i + 1 < z.nx) {
+                    const bool o = sOri[ly][lx + 1] & 1;
+                    nL = nL || o;
+                    nR = nR || !o;
+                }
+                face_put<1, 1, FY * (FX + 1)>(g, &sTx[ly][lx + 2], nL, nR, &sFx[0][ly][lx + 1]);
             }
-            const double* R = &sG[bf][ly + 3][lx];
-            const double* TX = &sTx[bf][ly][lx];
-            const double dgx = dweno_t(R[0], R[1], R[2], R[3], R[4], R[5], R[6], TX[0], TX[1], TX[2], TX[3], TX[4],
-                                       ue, z.inv6dx);
-            const double dgy = dweno_t(sG[bf][ly][lx + 3], sG[bf][ly + 1][lx + 3], sG[bf][ly + 2][lx + 3], gc,
-                                       sG[bf][ly + 4][lx + 3], sG[bf][ly + 5][lx + 3], sG[bf][ly + 6][lx + 3],
-                                       sTy[bf][ly][lx], sTy[bf][ly + 1][lx], sTy[bf][ly + 2][lx],
-                                       sTy[bf][ly + 3][lx], sTy[bf][ly + 4][lx], ve, z.inv6dy);
-            const double dgz = dweno_t(zG[0], zG[1], zG[2], gc, zG[4], zG[5], zG[6], zt[0], zt[1], zt[2], zt[3],
-                                       zt[4], we, z.inv6dz);
-            const double rr = -fma(we, dgz, fma(ue, dgx, ve * dgy));
+            {  // y face j+1/2
+                bool nL = Ly, nR = !Ly;
+                if (ly + 1 < FY && j + 1 < z.ny) {
+                    const bool o = (sOri[ly + 1][lx] >> 1) & 1;
+                    nL = nL || o;
+                    nR = nR || !o;
+                }
+                face_put<FGW, FX, (FY + 1) * FX>(g, &sTy[ly + 2][lx], nL, nR, &sFy[0][ly + 1][lx]);
+            }
+            // z face k-1/2 in this cell's orientation (carried unless it differs)
+            if (zlo_L != (int)Lz) {
+                if (Lz) zlo = face_core<false>(zG[0], zG[1], zG[2], zG[3], zG[4], zt[0], zt[1], zt[2]);
+                else zlo = face_core<true>(zG[5], zG[4], zG[3], zG[2], zG[1], zt[3], zt[2], zt[1]);
+            }
+            // z face k+1/2
+            if (Lz) zhi = face_core<false>(zG[1], zG[2], zG[3], zG[4], zG[5], zt[1], zt[2], zt[3]);
+            else zhi = face_core<true>(zG[6], zG[5], zG[4], zG[3], zG[2], zt[4], zt[3], zt[2]);
+        }
+        if (ly == 0 && lx < FY) {  // x face on the low edge of row lx, in its tile cell's orientation
+            if (y0 + lx < z.ny) {
+                const bool o = sOri[lx][0] & 1;
+                face_put<1, 1, FY * (FX + 1)>(&sG[lx + 3][2], &sTx[lx][1], o, !o, &sFx[0][lx][0]);
+            }
+        } else if (ly == 1) {  // y face on the low edge of column lx
+            if (x0 + lx < z.nx) {
+                const bool o = (sOri[0][lx] >> 1) & 1;
+                face_put<FGW, FX, (FY + 1) * FX>(&sG[2][lx + 3], &sTy[1][lx], o, !o, &sFy[0][0][lx]);
+            }
+        }
+        __syncthreads();  // faces
+        // --- derivatives, RHS, stage combine
+        if (active) {
+            const int ox = Lx ? 0 : 1, oy = Ly ? 0 : 1;
+            const double2 xl = sFx[ox][ly][lx], xh = sFx[ox][ly][lx + 1];
+            const double2 yl = sFy[oy][ly][lx], yh = sFy[oy][ly + 1][lx];
+            const double Nx = fma(xh.x, xl.y, -(xl.x * xh.y)), Dx = xh.y * xl.y;
+            const double Ny = fma(yh.x, yl.y, -(yl.x * yh.y)), Dy = yh.y * yl.y;
+            const double Nz = fma(zhi.n, zlo.d, -(zlo.n * zhi.d)), Dz = zhi.d * zlo.d;
+            const double Dxy = Dx * Dy;
+            const double num = fma((ue * z.inv6dx) * Nx, Dy * Dz,
+                                   fma((ve * z.inv6dy) * Ny, Dx * Dz, ((we * z.inv6dz) * Nz) * Dxy));
+            const double rr = -(num * rcp_nr(Dxy * Dz));
+            const double gc = zG[3];
             double o;
             if (STAGE == 1) o = fma(dt, rr, cgn);
-            else if (STAGE == 2) o = fma(0.25 * dt, rr, fma(0.25, gc, 0.75 * cgn));
-            else o = fma((2.0 / 3.0) * dt, rr, fma(2.0 / 3.0, gc, (1.0 / 3.0) * cgn));
-            out[c] = o;
-            if (z.fuse_ghosts && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3)) {
-                const int nxi = n_images(i, z.nx, z.bc[0]);
-                const int nyi = n_images(j, z.ny, z.bc[1]);
-                const int nzi = n_images(k, z.nz, z.bc[2]);
-                for (int a = 0; a < nzi; a++) {
-                    const int64_t pk = (int64_t)image(k, z.nz, z.bc[2], a) * sz;
-                    for (int b = 0; b < nyi; b++) {
-                        const int64_t pj = pk + (int64_t)image(j, z.ny, z.bc[1], b) * sy;
-                        for (int e = 0; e < nxi; e++)
-                            if (a | b | e) out[pj + image(i, z.nx, z.bc[0], e)] = o;
-                    }
-                }
-            }
+            else if (STAGE == 2) o = fma(c2, rr, fma(0.25, gc, 0.75 * cgn));
+            else o = fma(c3, rr, fma(2.0 / 3.0, gc, (1.0 / 3.0) * cgn));
+            out[plane + col] = o;
+            if (z.fuse_ghosts && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3))
+                store_images(z, out, i, j, k, o);
         }
+        zlo = zhi;
+        zlo_L = (int)Lz;
+        if (k + 1 < ke) put_plane(zG[4], nx);  // after the face barrier: sG is free
 #pragma unroll
         for (int q = 0; q < 6; q++) zG[q] = zG[q + 1];
 #pragma unroll
@@ -574,7 +668,7 @@ static FastGeom make_fast(const Geom& g) {
     return f;
 }
 
-static ZGeom make_z(const Geom& g, int kz) {
+static ZGeom make_z(const Geom& g, int kz, int tile_cols = ZX, int tile_rows = ZY) {
     const FastGeom f = make_fast(g);
     ZGeom z;
     z.nx = g.nx;
@@ -583,8 +677,8 @@ static ZGeom make_z(const Geom& g, int kz) {
     z.sy = (int32_t)g.sy;
     z.sz = (int32_t)g.sz;
     for (int a = 0; a < 3; a++) z.bc[a] = g.bc[a];
-    z.tiles_x = (g.nx + ZX - 1) / ZX;
-    z.tiles_y = (g.ny + ZY - 1) / ZY;
+    z.tiles_x = (g.nx + tile_cols - 1) / tile_cols;
+    z.tiles_y = (g.ny + tile_rows - 1) / tile_rows;
     z.kz = kz;
     const int nchunks = (g.nz + kz - 1) / kz;
     z.ntiles = (int64_t)z.tiles_x * z.tiles_y * nchunks;
@@ -611,15 +705,15 @@ int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double
     const int kz = g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16);
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
         return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
-    const ZGeom z = make_z(g, kz);
+    const ZGeom z = make_z(g, kz, FX, FY);
     if (z.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
-    const dim3 grid((unsigned)z.nblocks), block(ZX, ZY, 1);
+    const dim3 grid((unsigned)z.nblocks), block(FX, FY, 1);
     if (stage == 1)
-        hipLaunchKernelGGL((k_stage_zmarch<1>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        hipLaunchKernelGGL((k_stage_zface<1>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
     else if (stage == 2)
-        hipLaunchKernelGGL((k_stage_zmarch<2>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        hipLaunchKernelGGL((k_stage_zface<2>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
     else
-        hipLaunchKernelGGL((k_stage_zmarch<3>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        hipLaunchKernelGGL((k_stage_zface<3>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "z-march stage launch failed");
     return z.fuse_ghosts ? 1 : 0;
 }
