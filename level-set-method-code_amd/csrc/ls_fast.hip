@@ -192,6 +192,9 @@ __global__ __launch_bounds__(256) void k_hcr2_fast(FastGeom f, const double* __r
 // cells of the output are written by the interior cell they copy (periodic /
 // zero-gradient images), which replaces the separate ghost-fill launch.
 constexpr int ZX = 64, ZY = 4, ZH = 3;
+#ifndef LS_ZMARCH_KZ
+#define LS_ZMARCH_KZ 32  // z planes per block at large nz
+#endif
 
 // Kernel arguments of the z-marching kernels, kept small so that they stay in
 // scalar registers.
@@ -523,6 +526,51 @@ struct HGeom {
     double dx2, invdx, invdy, invdz, dtau, beta;
 };
 
+// One HCR-2 pseudo-time update of a cell (reinitialization.h:136-193, 3D) from
+// phi_n (P) and phi_0 (Q) at the cell and its neighbours in flag order
+// x-, x+, y-, y+, z-, z+ (reinitialization.h:46-89).  Crossing flags and r~
+// come from phi_0 with the arithmetic of computeInterfaceCrossings.
+__device__ __forceinline__ double hcr2_point(const HGeom& hg, const double (&P)[7], const double (&Q)[7]) {
+    const double pc = Q[0];
+    double sum0 = 0.0;
+    int flag = 0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        if (pc * Q[q + 1] < 0.0) {
+            sum0 += Q[q + 1];
+            flag |= 1 << q;
+        }
+    }
+    const double phi = P[0];
+    const double S = pc * rsqrt_nr(fma(pc, pc, hg.dx2));
+    const double xm = (P[0] - P[1]) * hg.invdx, xp = (P[2] - P[0]) * hg.invdx;
+    const double ym = (P[0] - P[3]) * hg.invdy, yp = (P[4] - P[0]) * hg.invdy;
+    const double zm = (P[0] - P[5]) * hg.invdz, zp = (P[6] - P[0]) * hg.invdz;
+    const bool pos = S > 0.0;
+    const double ax = pos ? fmax(xm, 0.0) : fmin(xm, 0.0), bx = pos ? fmin(xp, 0.0) : fmax(xp, 0.0);
+    const double ay = pos ? fmax(ym, 0.0) : fmin(ym, 0.0), by = pos ? fmin(yp, 0.0) : fmax(yp, 0.0);
+    const double az = pos ? fmax(zm, 0.0) : fmin(zm, 0.0), bz = pos ? fmin(zp, 0.0) : fmax(zp, 0.0);
+    const double s2 = (fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by)) + fmax(az * az, bz * bz);
+    const double term = -S * (sqrt(s2) - 1.0);
+    double forcing = 0.0;
+    if (flag != 0) {
+        bool stable = (P[0] > 0.0) == (pc > 0.0);
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+            if ((flag >> q) & 1) stable = stable && ((P[q + 1] > 0.0) == (Q[q + 1] > 0.0));
+        if (stable) {
+            double sum = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                if ((flag >> q) & 1) sum += P[q + 1];
+            // r~ = phi_0 / sum of crossing neighbours, 0 when that sum vanishes
+            const double rt = fabs(sum0) > 1e-15 ? div_fast(pc, sum0) : 0.0;
+            forcing = hg.beta * fma(rt, sum, -phi) * hg.invdx;
+        }
+    }
+    return fma(hg.dtau, term + forcing, phi);
+}
+
 template <int W, int H>
 __device__ __forceinline__ void h_load_tile(const HGeom& hg, int64_t plane, int x0, int y0, int tid,
                                             const double* __restrict__ F, double (&s)[H + 2][W + 2],
@@ -589,63 +637,145 @@ __global__ __launch_bounds__(256) void k_hcr2_zmarch(HGeom hg, const double* __r
         if (active) {
             const double P[7] = {a1, sP[ly + 1][lx], sP[ly + 1][lx + 2], sP[ly][lx + 1], sP[ly + 2][lx + 1], a0, a2};
             const double Q[7] = {b1, sQ[ly + 1][lx], sQ[ly + 1][lx + 2], sQ[ly][lx + 1], sQ[ly + 2][lx + 1], b0, b2};
-            // neighbours in flag order x-, x+, y-, y+, z-, z+ (reinitialization.h:46-89)
-            const double pc = Q[0];
-            double sum0 = 0.0;
-            int flag = 0;
-#pragma unroll
-            for (int q = 0; q < 6; q++) {
-                if (pc * Q[q + 1] < 0.0) {
-                    sum0 += Q[q + 1];
-                    flag |= 1 << q;
-                }
-            }
-            const double phi = P[0];
-            const double S = pc * rsqrt_nr(fma(pc, pc, hg.dx2));
-            const double xm = (P[0] - P[1]) * hg.invdx, xp = (P[2] - P[0]) * hg.invdx;
-            const double ym = (P[0] - P[3]) * hg.invdy, yp = (P[4] - P[0]) * hg.invdy;
-            const double zm = (P[0] - P[5]) * hg.invdz, zp = (P[6] - P[0]) * hg.invdz;
-            const bool pos = S > 0.0;
-            const double ax = pos ? fmax(xm, 0.0) : fmin(xm, 0.0), bx = pos ? fmin(xp, 0.0) : fmax(xp, 0.0);
-            const double ay = pos ? fmax(ym, 0.0) : fmin(ym, 0.0), by = pos ? fmin(yp, 0.0) : fmax(yp, 0.0);
-            const double az = pos ? fmax(zm, 0.0) : fmin(zm, 0.0), bz = pos ? fmin(zp, 0.0) : fmax(zp, 0.0);
-            const double s2 = (fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by)) + fmax(az * az, bz * bz);
-            const double term = -S * (sqrt(s2) - 1.0);
-            double forcing = 0.0;
-            if (flag != 0) {
-                bool stable = (P[0] > 0.0) == (pc > 0.0);
-#pragma unroll
-                for (int q = 0; q < 6; q++)
-                    if ((flag >> q) & 1) stable = stable && ((P[q + 1] > 0.0) == (Q[q + 1] > 0.0));
-                if (stable) {
-                    double sum = 0.0;
-#pragma unroll
-                    for (int q = 0; q < 6; q++)
-                        if ((flag >> q) & 1) sum += P[q + 1];
-                    const double rt = fabs(sum0) > 1e-15 ? div_fast(pc, sum0) : 0.0;
-                    forcing = hg.beta * fma(rt, sum, -phi) * hg.invdx;
-                }
-            }
-            const double o = fma(hg.dtau, term + forcing, phi);
+            const double o = hcr2_point(hg, P, Q);
             pnew[plane + col] = o;
-            if (z.fuse_ghosts && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3)) {
-                const int nxi = n_images(i, z.nx, z.bc[0]);
-                const int nyi = n_images(j, z.ny, z.bc[1]);
-                const int nzi = n_images(k, z.nz, z.bc[2]);
-                for (int a = 0; a < nzi; a++) {
-                    const int64_t pk = (int64_t)image(k, z.nz, z.bc[2], a) * sz;
-                    for (int b = 0; b < nyi; b++) {
-                        const int64_t pj = pk + (int64_t)image(j, z.ny, z.bc[1], b) * sy;
-                        for (int e = 0; e < nxi; e++)
-                            if (a | b | e) pnew[pj + image(i, z.nx, z.bc[0], e)] = o;
-                    }
-                }
-            }
+            if (z.fuse_ghosts && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3))
+                store_images(z, pnew, i, j, k, o);
         }
         a0 = a1;
         a1 = a2;
         b0 = b1;
         b1 = b2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The first two HCR-2 iterations in one z-marching pass.  The first iteration
+// has phi_n = phi_0, so phi_1 = step(phi_0, phi_0) needs phi_0 only, and
+// phi_2 = step(phi_1, phi_0) needs phi_1 at the cell and its six neighbours.
+// Plane k computes phi_1 on plane k+1 over the tile plus a one-cell ring
+// (into LDS), then phi_2 on plane k.  The phi_0 planes k-2..k+2 (tile plus a
+// two-cell ring) sit in a five-plane LDS ring, so a plane needs two barriers.
+// phi_1 at a ghost position takes the value the ghost fill between the two
+// iterations would give it (reinitialization.h:349-355):
+//  * a periodic axis computes it in place, because phi_0 is periodic over the
+//    padded array, so the stencil there equals the stencil of the image;
+//  * a zero-gradient axis takes it from the clamped cell.
+// Both iterations use hcr2_point, so the result is bitwise that of two
+// single iterations with a ghost fill between them.  Needs the ghost images
+// of the output (no linear-extrapolation axis).
+constexpr int PQX = HX + 4, PQY = HY + 4;                      // phi_0 plane: tile + 2-cell ring
+constexpr int P1X = HX + 2, P1Y = HY + 2;                      // phi_1 plane: tile + 1-cell ring
+constexpr int kQItems = PQX * PQY, kQPer = (kQItems + HX * HY - 1) / (HX * HY);
+constexpr int kP1Items = P1X * P1Y, kP1Per = (kP1Items + HX * HY - 1) / (HX * HY);
+
+__global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __restrict__ p0,
+                                                    double* __restrict__ pnew) {
+    __shared__ double sQ[5][PQY][PQX];   // phi_0 plane kk in slot (kk + 10) % 5 (kk >= -2)
+    __shared__ double sP1[2][P1Y][P1X];  // phi_1 plane kk in slot kk & 1
+    const ZGeom& z = hg.z;
+    const int64_t t = xcd_tile(blockIdx.x, z.nblocks);
+    if (t >= z.ntiles) return;  // block-uniform
+    const int tx = (int)(t % z.tiles_x);
+    const int64_t r = t / z.tiles_x;
+    const int ty = (int)(r % z.tiles_y);
+    const int ch = (int)(r / z.tiles_y);
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * HX + lx;
+    const int x0 = tx * HX, y0 = ty * HY;
+    const int i = x0 + lx, j = y0 + ly;
+    const bool active = i < z.nx && j < z.ny;
+    const int kb = ch * z.kz;
+    const int ke = min(kb + z.kz, z.nz);
+    const int64_t sz = z.sz;
+    const bool zgx = z.bc[0] == LS_BC_ZERO_GRAD, zgy = z.bc[1] == LS_BC_ZERO_GRAD, zgz = z.bc[2] == LS_BC_ZERO_GRAD;
+
+    // phi_0 items: local (x, y) in [-2, HX+1] x [-2, HY+1]; positions past the
+    // padded array read the nearest padded cell (they feed no interior cell)
+    int qx[kQPer], qy[kQPer];
+    int32_t qoff[kQPer];
+#pragma unroll
+    for (int a = 0; a < kQPer; a++) {
+        const int e = min(tid + a * HX * HY, kQItems - 1);
+        qy[a] = e / PQX;
+        qx[a] = e - qy[a] * PQX;
+        const int gx = min(x0 - 2 + qx[a], z.nx + 2), gy = min(y0 - 2 + qy[a], z.ny + 2);
+        qoff[a] = (gy + 3) * z.sy + gx + 3;
+    }
+    auto slot = [](int kk) { return (kk + 10) % 5; };
+    auto load_q = [&](int kk, double (&val)[kQPer]) {
+        const int64_t pl = (int64_t)(kk + 3) * sz;
+#pragma unroll
+        for (int a = 0; a < kQPer; a++) val[a] = p0[pl + qoff[a]];
+    };
+    auto put_q = [&](int kk, const double (&val)[kQPer]) {
+        const int sl = slot(kk);
+#pragma unroll
+        for (int a = 0; a < kQPer; a++)
+            if (tid + a * HX * HY < kQItems) sQ[sl][qy[a]][qx[a]] = val[a];
+    };
+    // phi_1 items: local (x, y) in [-1, HX] x [-1, HY], evaluated at (ex, ey)
+    // (sQ indices; the clamped cell on a zero-gradient axis)
+    int px[kP1Per], py[kP1Per], ex[kP1Per], ey[kP1Per];
+#pragma unroll
+    for (int a = 0; a < kP1Per; a++) {
+        const int e = min(tid + a * HX * HY, kP1Items - 1);
+        py[a] = e / P1X;
+        px[a] = e - py[a] * P1X;
+        int gx = x0 - 1 + px[a], gy = y0 - 1 + py[a];
+        if (zgx) gx = min(max(gx, 0), z.nx - 1);
+        if (zgy) gy = min(max(gy, 0), z.ny - 1);
+        ex[a] = gx - x0 + 2;
+        ey[a] = gy - y0 + 2;
+    }
+    auto zev = [&](int kk) { return zgz ? min(max(kk, 0), z.nz - 1) : kk; };
+    // phi_1 on plane kk at sQ position (X, Y): the first iteration, phi_n = phi_0
+    auto phi1 = [&](int kk, int X, int Y) {
+        const int sc = slot(kk), sm = slot(kk - 1), sp = slot(kk + 1);
+        const double Q[7] = {sQ[sc][Y][X],     sQ[sc][Y][X - 1], sQ[sc][Y][X + 1], sQ[sc][Y - 1][X],
+                             sQ[sc][Y + 1][X], sQ[sm][Y][X],     sQ[sp][Y][X]};
+        return hcr2_point(hg, Q, Q);
+    };
+
+    double val[kQPer];
+    for (int kk = kb - 2; kk <= kb + 1; kk++) {
+        load_q(kk, val);
+        put_q(kk, val);
+    }
+    load_q(kb + 2, val);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < kP1Per; a++)
+        if (tid + a * HX * HY < kP1Items) sP1[kb & 1][py[a]][px[a]] = phi1(zev(kb), ex[a], ey[a]);
+    double fm = phi1(zev(kb - 1), lx + 2, ly + 2);  // phi_1 of this column one plane below
+
+    for (int k = kb; k < ke; k++) {
+        put_q(k + 2, val);  // slot of plane k-3, last read two planes ago
+        if (k + 1 < ke) load_q(k + 3, val);
+        __syncthreads();  // phi_0 plane k+2 in place; phi_1 plane k-1 no longer read
+#pragma unroll
+        for (int a = 0; a < kP1Per; a++)
+            if (tid + a * HX * HY < kP1Items) sP1[(k + 1) & 1][py[a]][px[a]] = phi1(zev(k + 1), ex[a], ey[a]);
+        __syncthreads();  // phi_1 plane k+1 in place
+        if (active) {
+            const int s0 = k & 1, X = lx + 1, Y = ly + 1;
+            const double fc = sP1[s0][Y][X];
+            const double P[7] = {fc,
+                                 sP1[s0][Y][X - 1],
+                                 sP1[s0][Y][X + 1],
+                                 sP1[s0][Y - 1][X],
+                                 sP1[s0][Y + 1][X],
+                                 fm,
+                                 sP1[(k + 1) & 1][Y][X]};
+            const int sc = slot(k), QX = lx + 2, QY = ly + 2;
+            const double Q[7] = {sQ[sc][QY][QX],     sQ[sc][QY][QX - 1], sQ[sc][QY][QX + 1], sQ[sc][QY - 1][QX],
+                                 sQ[sc][QY + 1][QX], sQ[slot(k - 1)][QY][QX], sQ[slot(k + 1)][QY][QX]};
+            const double o = hcr2_point(hg, P, Q);
+            const int64_t c = (int64_t)(k + 3) * sz + (int64_t)(j + 3) * z.sy + (i + 3);
+            pnew[c] = o;
+            if (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3)
+                store_images(z, pnew, i, j, k, o);
+            fm = fc;
+        }
     }
 }
 
@@ -699,9 +829,6 @@ static ZGeom make_z(const Geom& g, int kz, int tile_cols = ZX, int tile_rows = Z
 int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
                         const double* u, const double* v, const double* w, const double* sl, double dt,
                         hipStream_t st) {
-#ifndef LS_ZMARCH_KZ
-#define LS_ZMARCH_KZ 32
-#endif
     const int kz = g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16);
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
         return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
@@ -740,6 +867,28 @@ template int launch_stage_fast<2>(const Geom&, int, const double*, const double*
                                   const double*, const double*, const double*, double, hipStream_t);
 template int launch_stage_fast<3>(const Geom&, int, const double*, const double*, double*, const double*,
                                   const double*, const double*, const double*, double, hipStream_t);
+
+// The first two HCR-2 iterations from phi_0 in one launch (3D).  Returns 1
+// when it ran (the output's ghost cells included), 0 when this grid needs the
+// single-iteration path (a linear-extrapolation axis).
+int launch_hcr2_pair_fast(const Geom& g, const double* p0, double* pnew, hipStream_t st) {
+    if (g.dim != 3) return 0;
+    if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
+        return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
+    HGeom hg;
+    hg.z = make_z(g, g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16));
+    if (!hg.z.fuse_ghosts) return 0;
+    hg.dx2 = g.dx * g.dx;
+    hg.invdx = 1.0 / g.dx;
+    hg.invdy = 1.0 / g.dy;
+    hg.invdz = 1.0 / g.dz;
+    hg.dtau = g.dtau;
+    hg.beta = g.beta;
+    if (hg.z.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
+    hipLaunchKernelGGL(k_hcr2_pair0, dim3((unsigned)hg.z.nblocks), dim3(HX, HY, 1), 0, st, hg, p0, pnew);
+    if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast HCR-2 pair launch failed");
+    return 1;
+}
 
 // Returns 1 when the kernel also filled the ghost cells of pnew.
 int launch_hcr2_fast(const Geom& g, const double* pn, const double* p0, double* pnew, hipStream_t st) {

@@ -11,4 +11,5 @@ int launch_stage_fast(const lsx::Geom& g, int stage, const double* Gs, const dou
                       const double* sl, double dt, hipStream_t st);
 int launch_hcr2_fast(const lsx::Geom& g, const double* pn, const double* p0, double* pnew,
                      hipStream_t st);
+int launch_hcr2_pair_fast(const lsx::Geom& g, const double* p0, double* pnew, hipStream_t st);
 }  // namespace lsf

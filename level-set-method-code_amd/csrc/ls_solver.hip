@@ -392,12 +392,22 @@ static int hcr2_rotate(const ls_solver* s, double** cur, double** a, double** b,
     double* p0 = *cur;
     double* src = p0;
     double* dst = *a;
-    double* other = *b;
-    for (int it = 0; it < iters; it++) {
+    int it = 0;
+    if (s->p.mode == LS_MODE_FAST && iters >= 2) {
+        // the first two iterations in one pass where the grid allows it
+        int rc = lsf::launch_hcr2_pair_fast(s->g, p0, *b, st);
+        if (rc < 0) return rc;
+        if (rc == 1) {
+            src = *b;
+            dst = *a;
+            it = 2;
+        }
+    }
+    for (; it < iters; it++) {
         int rc = hcr2_iter(s, src, p0, dst, st);
         if (rc) return rc;
         src = dst;
-        dst = (it == 0) ? other : (src == *a ? *b : *a);
+        dst = (src == *a) ? *b : *a;
     }
     // src holds the result; the two other buffers of {p0, a, b} are free.
     double* bufs[3] = {p0, *a, *b};

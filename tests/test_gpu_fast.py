@@ -153,3 +153,28 @@ def test_fast_log_known_answers(ls, cuda):
     steps, _, _ = s3.advance(dG, _dev(cuda, uu), _dev(cuda, vv), _dev(cuda, ww), dt=0.001, t_final=1.5)
     l2, vol = s3.diag(dG, dG0)
     assert steps == 1501 and "%.6e" % l2 == "9.158634e-04" and "%.6f" % vol == "0.014245"
+
+
+@pytest.mark.parametrize("n,bc", [((70, 9, 40), (1, 1, 1)),    # partial tiles, zero-gradient images
+                                  ((66, 13, 37), (0, 0, 0)),   # periodic, odd chunk lengths
+                                  ((64, 8, 70), (0, 1, 1)),    # mixed axes, z chunking
+                                  ((130, 6, 5), (1, 0, 0))])   # three x tiles, ny and nz below a tile
+@pytest.mark.parametrize("iters", [2, 3])
+def test_fast_hcr2_pair_bitwise(ls, oracle, cuda, n, bc, iters):
+    """The first two HCR-2 iterations run as one pass over phi_0.  That pass
+    must give bit for bit the two single iterations with a ghost fill between
+    them (ls_hcr2_step), over the whole padded array, ghost cells included."""
+    g = make_grid(3, *n, bc=bc, s_l=0.0)
+    p = ls.default_params(3, n[0], nx=n[0], ny=n[1], nz=n[2], bc=bc, mode=ls.MODE_FAST, reinit_iters=iters)
+    G = oracle.fill_ghosts(g, 1.7 * bumpy_sdf(g.shape, (p.dz, p.dy, p.dx), 23, noise=0.05))
+    s = ls.Solver(p)
+    fused = _dev(cuda, G)
+    s.reinit(fused)
+    phi0 = _dev(cuda, G)
+    cur = phi0
+    for _ in range(iters):
+        nxt = s.empty()
+        s.hcr2_step(cur, phi0, nxt)
+        cur = nxt
+    assert not cuda.equal(cur, phi0)
+    assert cuda.equal(fused, cur)
