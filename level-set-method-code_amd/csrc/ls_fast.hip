@@ -543,14 +543,17 @@ __device__ __forceinline__ double hcr2_point(const HGeom& hg, const double (&P)[
     }
     const double phi = P[0];
     const double S = pc * rsqrt_nr(fma(pc, pc, hg.dx2));
-    const double xm = (P[0] - P[1]) * hg.invdx, xp = (P[2] - P[0]) * hg.invdx;
-    const double ym = (P[0] - P[3]) * hg.invdy, yp = (P[4] - P[0]) * hg.invdy;
-    const double zm = (P[0] - P[5]) * hg.invdz, zp = (P[6] - P[0]) * hg.invdz;
+    // Godunov |grad phi| (weno5.h:202-248): for S > 0 an axis contributes
+    // max(max(D-, 0)^2, min(D+, 0)^2), for S <= 0 the mirrored min/max.  With
+    // the sign of S folded into 1/h both read max(A, B)^2 over the
+    // non-negative A = max(+-D-, 0), B = max(-+D+, 0), which is the same value
+    // bit for bit (A^2 >= B^2 exactly when A >= B).
     const bool pos = S > 0.0;
-    const double ax = pos ? fmax(xm, 0.0) : fmin(xm, 0.0), bx = pos ? fmin(xp, 0.0) : fmax(xp, 0.0);
-    const double ay = pos ? fmax(ym, 0.0) : fmin(ym, 0.0), by = pos ? fmin(yp, 0.0) : fmax(yp, 0.0);
-    const double az = pos ? fmax(zm, 0.0) : fmin(zm, 0.0), bz = pos ? fmin(zp, 0.0) : fmax(zp, 0.0);
-    const double s2 = (fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by)) + fmax(az * az, bz * bz);
+    const double hx = pos ? hg.invdx : -hg.invdx, hy = pos ? hg.invdy : -hg.invdy, hz = pos ? hg.invdz : -hg.invdz;
+    const double gx = fmax(fmax((P[0] - P[1]) * hx, 0.0), fmax((P[0] - P[2]) * hx, 0.0));
+    const double gy = fmax(fmax((P[0] - P[3]) * hy, 0.0), fmax((P[0] - P[4]) * hy, 0.0));
+    const double gz = fmax(fmax((P[0] - P[5]) * hz, 0.0), fmax((P[0] - P[6]) * hz, 0.0));
+    const double s2 = (gx * gx + gy * gy) + gz * gz;
     const double term = -S * (sqrt(s2) - 1.0);
     double forcing = 0.0;
     if (flag != 0) {
