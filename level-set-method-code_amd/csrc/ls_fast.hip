@@ -306,12 +306,17 @@ __device__ __forceinline__ void store_images(const ZGeom& z, double* __restrict_
     }
 }
 
-// Tile of the shared-face kernel: 64 x 4 cells per plane, one cell per
-// thread, one wave per row.  Besides its row, each wave takes a share of the
-// tile's edge work: every wave one row of the y-edge smoothness terms, wave 2
-// the x-edge terms, wave 0 the x faces and wave 1 the y faces on the low
-// tile edges.
-constexpr int FX = 64, FY = 4;
+// Tile of the shared-face kernel: 64 x FY cells per plane, one cell per
+// thread, one wave per row.  Besides its row, a wave may take a share of the
+// tile's edge work: waves 0-3 the y-edge smoothness terms (one row each),
+// wave kWaveTx the x-edge terms, waves kWaveFx and kWaveFy the x and y faces
+// on the low tile edges.
+#ifndef LS_ZFACE_ROWS
+#define LS_ZFACE_ROWS 4
+#endif
+constexpr int FX = 64, FY = LS_ZFACE_ROWS;
+static_assert(FY == 4 || FY == 8, "tile rows");
+constexpr int kWaveTx = FY == 4 ? 2 : 5, kWaveFx = FY == 4 ? 0 : 6, kWaveFy = FY == 4 ? 1 : 7;
 constexpr int FGH = FY + 2 * ZH, FGW = FX + 2 * ZH;
 constexpr int kFaceThreads = FX * FY;
 constexpr int kHaloItems = 6 * FX + 6 * FY;   // G halo values per plane
@@ -415,11 +420,11 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
         // --- smoothness terms along x and y, effective velocity, orientation
         sTx[ly][lx + 2] = tsq(sG[ly + 3][lx + 2], sG[ly + 3][lx + 3], sG[ly + 3][lx + 4]);
         sTy[ly + 2][lx] = tsq(sG[ly + 2][lx + 3], sG[ly + 3][lx + 3], sG[ly + 4][lx + 3]);
-        {
+        if (ly < 4) {
             const int ry = ly < 2 ? ly : ly + FY;  // local rows -2, -1, FY, FY+1
             sTy[ry][lx] = tsq(sG[ry][lx + 3], sG[ry + 1][lx + 3], sG[ry + 2][lx + 3]);
         }
-        if (ly == 2 && lx < 4 * FY) {  // local x -2, -1, FX, FX+1 of each row
+        if (ly == kWaveTx && lx < 4 * FY) {  // local x -2, -1, FX, FX+1 of each row
             const int row = lx >> 2, c4 = lx & 3;
             const int q = c4 < 2 ? c4 : c4 + FX;
             sTx[row][q] = tsq(sG[row + 3][q], sG[row + 3][q + 1], sG[row + 3][q + 2]);
@@ -470,12 +475,12 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
             if (Lz) zhi = face_core<false>(zG[1], zG[2], zG[3], zG[4], zG[5], zt[1], zt[2], zt[3]);
             else zhi = face_core<true>(zG[6], zG[5], zG[4], zG[3], zG[2], zt[4], zt[3], zt[2]);
         }
-        if (ly == 0 && lx < FY) {  // x face on the low edge of row lx, in its tile cell's orientation
+        if (ly == kWaveFx && lx < FY) {  // x face on the low edge of row lx, in its tile cell's orientation
             if (y0 + lx < z.ny) {
                 const bool o = sOri[lx][0] & 1;
                 face_put<1, 1, FY * (FX + 1)>(&sG[lx + 3][2], &sTx[lx][1], o, !o, &sFx[0][lx][0]);
             }
-        } else if (ly == 1) {  // y face on the low edge of column lx
+        } else if (ly == kWaveFy) {  // y face on the low edge of column lx
             if (x0 + lx < z.nx) {
                 const bool o = (sOri[0][lx] >> 1) & 1;
                 face_put<FGW, FX, (FY + 1) * FX>(&sG[2][lx + 3], &sTy[1][lx], o, !o, &sFy[0][0][lx]);
