@@ -49,8 +49,9 @@ HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
 STAGE_BYTES = {1: 40, 2: 48, 3: 48}   # algorithmic bytes per interior cell per stage
 STEP_BYTES = 136 + 83 / 5.0           # RK3 + amortised HCR-2, bytes per cell-update
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_traffic_512.json")
-# the 3D fast-mode stage kernel (shared-face z-march)
-FAST_STAGE_KERNEL = "k_stage_zface<2>"
+# the 3D stage kernels (shared-face z-march), fast and exact arithmetic
+FAST_STAGE_KERNEL = "k_stage_zface<2, false>"
+EXACT_STAGE_KERNEL = "k_stage_zface<2, true>"
 
 
 def load_ls():
@@ -386,7 +387,7 @@ def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=TRAFFIC_FILE):
     wr = per_kernel(_pmc_rows(write_dir), "WRITE_SIZE")
     copy_k = [k for k in fr if "MulFunctor" in k]
     stage_k = [k for k in fr if ((mode == "fast" and FAST_STAGE_KERNEL in k) or
-                                 (mode == "exact" and "k_stage_exact<3, 2>" in k))]
+                                 (mode == "exact" and EXACT_STAGE_KERNEL in k))]
     if not copy_k or not stage_k:
         raise SystemExit("kernels not found in pmc output: %s" % list(fr))
     ck, sk = copy_k[0], stage_k[0]
@@ -476,7 +477,7 @@ def main():
     traffic = read_traffic(a.n, a.mode)
     roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": ach / HBM_PEAK, "traffic": traffic,
-            "kernel": (FAST_STAGE_KERNEL if a.mode == "fast" else "k_stage_exact<3,2>") +
+            "kernel": (FAST_STAGE_KERNEL if a.mode == "fast" else EXACT_STAGE_KERNEL) +
                       " (fused WENO5/u_eff RHS + stage-2 combine)",
             "kernel_ms": t_stage * 1e3, "ghost_fill_ms": t_fill * 1e3,
             "algorithmic_bytes_per_cell": STAGE_BYTES[2],

@@ -23,6 +23,8 @@
 //  * constant divisions (by 6, 2h, h) become multiplies, FMA is used freely.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ls_fast.hpp"
 #include "ls_fast_math.hpp"
 #include "ls_internal.hpp"
@@ -206,6 +208,7 @@ struct ZGeom {
     int64_t ntiles, nblocks;
     double s_l, eps;
     double inv2dx, inv2dy, inv2dz, inv6dx, inv6dy, inv6dz;
+    double dx, dy, dz;            // exact-mode stage kernel: the reference divides by these
 };
 
 // The a-th padded index that holds interior index q after a ghost fill
@@ -322,16 +325,38 @@ constexpr int kFaceThreads = FX * FY;
 constexpr int kHaloItems = 6 * FX + 6 * FY;   // G halo values per plane
 constexpr int NH = (kHaloItems + kFaceThreads - 1) / kFaceThreads;  // per thread
 
-template <int STAGE>
+// A stored face as the value (exact mode) or the (n, d) pair (fast mode);
+// each overload is called only in its own mode.
+__device__ __forceinline__ double val(double f) { return f; }
+__device__ __forceinline__ double val(const double2& f) { return f.x; }
+__device__ __forceinline__ double2 as2(const double2& f) { return f; }
+__device__ __forceinline__ double2 as2(double f) { return make_double2(f, 1.0); }
+
+// Exact-mode faces: the reference's weno5_left on G(c-2..c+2) and
+// weno5_right on G(c-1..c+3) for the face c+1/2 (weno5.h:39-81), in its own
+// operation order.  A face value depends only on those five values, so the
+// two cells beside a face get the same bits whether it is built once or
+// twice.  g points at G(c) with stride GS.
+template <int GS>
+__device__ __forceinline__ double xface_lds(const double* g, bool mirror) {
+    return mirror ? lsx::weno_r(g[-GS], g[0], g[GS], g[2 * GS], g[3 * GS])
+                  : lsx::weno_l(g[-2 * GS], g[-GS], g[0], g[GS], g[2 * GS]);
+}
+
+// EXACT: the exact mode's arithmetic (bitwise the CPU oracle) on the same
+// shared-face z-march; faces are then single values and the smoothness terms
+// are computed inside each reconstruction, as the reference does.
+template <int STAGE, bool EXACT>
 __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_stage_zface(
     ZGeom z, const double* __restrict__ Gs, const double* __restrict__ Gn, double* __restrict__ out,
     const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
     const double* __restrict__ sl, double dt) {
-    __shared__ double sG[FGH][FGW];            // plane k: tile + 3-cell halo
-    __shared__ double sTx[FY][FX + 4];         // T along x at local x = -2..FX+1
-    __shared__ double sTy[FY + 4][FX];         // T along y at local y = -2..FY+1
-    __shared__ double2 sFx[2][FY][FX + 1];     // [left-biased, mirrored] x face between local x s-1 and s
-    __shared__ double2 sFy[2][FY + 1][FX];     // [left-biased, mirrored] y face between local y s-1 and s
+    using FV = typename std::conditional<EXACT, double, double2>::type;  // a stored face
+    __shared__ double sG[FGH][FGW];                      // plane k: tile + 3-cell halo
+    __shared__ double sTx[EXACT ? 1 : FY][FX + 4];       // T along x at local x = -2..FX+1
+    __shared__ double sTy[EXACT ? 1 : FY + 4][FX];       // T along y at local y = -2..FY+1
+    __shared__ FV sFx[2][FY][FX + 1];                    // [left-biased, mirrored] x face between local x s-1 and s
+    __shared__ FV sFy[2][FY + 1][FX];                    // [left-biased, mirrored] y face between local y s-1 and s
     __shared__ unsigned char sOri[FY][FX];     // bit 0: u_eff >= 0, bit 1: v_eff >= 0
     const int64_t t = xcd_tile(blockIdx.x, z.nblocks);
     if (t >= z.ntiles) return;  // block-uniform
@@ -418,6 +443,7 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
         if (k + 1 < ke) fetch(k + 1, nx);
         __syncthreads();  // plane k in sG
         // --- smoothness terms along x and y, effective velocity, orientation
+        if (!EXACT) {
         sTx[ly][lx + 2] = tsq(sG[ly + 3][lx + 2], sG[ly + 3][lx + 3], sG[ly + 3][lx + 4]);
         sTy[ly + 2][lx] = tsq(sG[ly + 2][lx + 3], sG[ly + 3][lx + 3], sG[ly + 4][lx + 3]);
         if (ly < 4) {
@@ -429,8 +455,20 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
             const int q = c4 < 2 ? c4 : c4 + FX;
             sTx[row][q] = tsq(sG[row + 3][q], sG[row + 3][q + 1], sG[row + 3][q + 2]);
         }
+        }
         double ue = cu, ve = cv, we = cw;
-        if (csl > z.eps) {
+        if (EXACT) {  // computeRHS3D (rk3.h:31-81), C++ evaluation order
+            double gx = 0.0, gy = 0.0, gz = 0.0, gm = 1.0;
+            if (csl > z.eps) {
+                gx = (sG[ly + 3][lx + 4] - sG[ly + 3][lx + 2]) / (2.0 * z.dx);
+                gy = (sG[ly + 4][lx + 3] - sG[ly + 2][lx + 3]) / (2.0 * z.dy);
+                gz = (zG[4] - zG[2]) / (2.0 * z.dz);
+                gm = sqrt(gx * gx + gy * gy + gz * gz + z.eps);
+            }
+            ue = cu - csl * gx / gm;
+            ve = cv - csl * gy / gm;
+            we = cw - csl * gz / gm;
+        } else if (csl > z.eps) {
             const double gx = (sG[ly + 3][lx + 4] - sG[ly + 3][lx + 2]) * z.inv2dx;
             const double gy = (sG[ly + 4][lx + 3] - sG[ly + 2][lx + 3]) * z.inv2dy;
             const double gz = (zG[4] - zG[2]) * z.inv2dz;
@@ -444,6 +482,22 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
         sOri[ly][lx] = (unsigned char)((Lx ? 1 : 0) | (Ly ? 2 : 0));
         __syncthreads();  // T, orientations
         // --- faces
+        // face c+1/2 of an axis into slot[0] (left-biased) and slot[OS] (mirrored)
+        auto put = [&](auto gs, auto ts, auto os, const double* g, const double* t, bool nL, bool nR, FV* slot) {
+            constexpr int GS = decltype(gs)::value, TS = decltype(ts)::value, OS = decltype(os)::value;
+            if constexpr (EXACT) {
+                (void)t;
+                if (nL) slot[0] = xface_lds<GS>(g, false);
+                if (nR) slot[OS] = xface_lds<GS>(g, true);
+            } else {
+                face_put<GS, TS, OS>(g, t, nL, nR, slot);
+            }
+        };
+        using C1 = std::integral_constant<int, 1>;
+        using CGW = std::integral_constant<int, FGW>;
+        using CFX = std::integral_constant<int, FX>;
+        using COX = std::integral_constant<int, FY * (FX + 1)>;
+        using COY = std::integral_constant<int, (FY + 1) * FX>;
         Face zhi;
         zhi.n = zhi.d = 0.0;
         if (active) {
@@ -455,7 +509,8 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
                     nL = nL || o;
                     nR = nR || !o;
                 }
-                face_put<1, 1, FY * (FX + 1)>(g, &sTx[ly][lx + 2], nL, nR, &sFx[0][ly][lx + 1]);
+                put(C1(), C1(), COX(), g, &sTx[0][0] + (EXACT ? 0 : ly * (FX + 4) + lx + 2), nL, nR,
+                    &sFx[0][ly][lx + 1]);
             }
             {  // y face j+1/2
                 bool nL = Ly, nR = !Ly;
@@ -464,34 +519,58 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
                     nL = nL || o;
                     nR = nR || !o;
                 }
-                face_put<FGW, FX, (FY + 1) * FX>(g, &sTy[ly + 2][lx], nL, nR, &sFy[0][ly + 1][lx]);
+                put(CGW(), CFX(), COY(), g, &sTy[0][0] + (EXACT ? 0 : (ly + 2) * FX + lx), nL, nR,
+                    &sFy[0][ly + 1][lx]);
             }
-            // z face k-1/2 in this cell's orientation (carried unless it differs)
-            if (zlo_L != (int)Lz) {
-                if (Lz) zlo = face_core<false>(zG[0], zG[1], zG[2], zG[3], zG[4], zt[0], zt[1], zt[2]);
-                else zlo = face_core<true>(zG[5], zG[4], zG[3], zG[2], zG[1], zt[3], zt[2], zt[1]);
+            if constexpr (EXACT) {
+                // z faces as single values in zlo.n
+                if (zlo_L != (int)Lz)
+                    zlo.n = Lz ? lsx::weno_l(zG[0], zG[1], zG[2], zG[3], zG[4])
+                               : lsx::weno_r(zG[1], zG[2], zG[3], zG[4], zG[5]);
+                zhi.n = Lz ? lsx::weno_l(zG[1], zG[2], zG[3], zG[4], zG[5])
+                           : lsx::weno_r(zG[2], zG[3], zG[4], zG[5], zG[6]);
+            } else {
+                // z face k-1/2 in this cell's orientation (carried unless it differs)
+                if (zlo_L != (int)Lz) {
+                    if (Lz) zlo = face_core<false>(zG[0], zG[1], zG[2], zG[3], zG[4], zt[0], zt[1], zt[2]);
+                    else zlo = face_core<true>(zG[5], zG[4], zG[3], zG[2], zG[1], zt[3], zt[2], zt[1]);
+                }
+                // z face k+1/2
+                if (Lz) zhi = face_core<false>(zG[1], zG[2], zG[3], zG[4], zG[5], zt[1], zt[2], zt[3]);
+                else zhi = face_core<true>(zG[6], zG[5], zG[4], zG[3], zG[2], zt[4], zt[3], zt[2]);
             }
-            // z face k+1/2
-            if (Lz) zhi = face_core<false>(zG[1], zG[2], zG[3], zG[4], zG[5], zt[1], zt[2], zt[3]);
-            else zhi = face_core<true>(zG[6], zG[5], zG[4], zG[3], zG[2], zt[4], zt[3], zt[2]);
         }
         if (ly == kWaveFx && lx < FY) {  // x face on the low edge of row lx, in its tile cell's orientation
             if (y0 + lx < z.ny) {
                 const bool o = sOri[lx][0] & 1;
-                face_put<1, 1, FY * (FX + 1)>(&sG[lx + 3][2], &sTx[lx][1], o, !o, &sFx[0][lx][0]);
+                put(C1(), C1(), COX(), &sG[lx + 3][2], &sTx[0][0] + (EXACT ? 0 : lx * (FX + 4) + 1), o, !o,
+                    &sFx[0][lx][0]);
             }
         } else if (ly == kWaveFy) {  // y face on the low edge of column lx
             if (x0 + lx < z.nx) {
                 const bool o = (sOri[0][lx] >> 1) & 1;
-                face_put<FGW, FX, (FY + 1) * FX>(&sG[2][lx + 3], &sTy[1][lx], o, !o, &sFy[0][0][lx]);
+                put(CGW(), CFX(), COY(), &sG[2][lx + 3], &sTy[0][0] + (EXACT ? 0 : FX + lx), o, !o,
+                    &sFy[0][0][lx]);
             }
         }
         __syncthreads();  // faces
         // --- derivatives, RHS, stage combine
-        if (active) {
+        if (active && EXACT) {
+            // dweno_*_3d: (face(c+1/2) - face(c-1/2)) / h; rhs and stage in the
+            // reference's order (rk3.h:31-125)
             const int ox = Lx ? 0 : 1, oy = Ly ? 0 : 1;
-            const double2 xl = sFx[ox][ly][lx], xh = sFx[ox][ly][lx + 1];
-            const double2 yl = sFy[oy][ly][lx], yh = sFy[oy][ly + 1][lx];
+            const double dgx = (val(sFx[ox][ly][lx + 1]) - val(sFx[ox][ly][lx])) / z.dx;
+            const double dgy = (val(sFy[oy][ly + 1][lx]) - val(sFy[oy][ly][lx])) / z.dy;
+            const double dgz = (zhi.n - zlo.n) / z.dz;
+            const double rr = -(ue * dgx + ve * dgy + we * dgz);
+            const double o = lsx::stage_combine<STAGE>(cgn, zG[3], rr, dt);
+            out[plane + col] = o;
+            if (z.fuse_ghosts && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3))
+                store_images(z, out, i, j, k, o);
+        } else if (active) {
+            const int ox = Lx ? 0 : 1, oy = Ly ? 0 : 1;
+            const double2 xl = as2(sFx[ox][ly][lx]), xh = as2(sFx[ox][ly][lx + 1]);
+            const double2 yl = as2(sFy[oy][ly][lx]), yh = as2(sFy[oy][ly + 1][lx]);
             const double Nx = fma(xh.x, xl.y, -(xl.x * xh.y)), Dx = xh.y * xl.y;
             const double Ny = fma(yh.x, yl.y, -(yl.x * yh.y)), Dy = yh.y * yl.y;
             const double Nz = fma(zhi.n, zlo.d, -(zlo.n * zhi.d)), Dz = zhi.d * zlo.d;
@@ -830,34 +909,50 @@ static ZGeom make_z(const Geom& g, int kz, int tile_cols = ZX, int tile_rows = Z
     z.inv6dx = f.inv6dx;
     z.inv6dy = f.inv6dy;
     z.inv6dz = f.inv6dz;
+    z.dx = g.dx;
+    z.dy = g.dy;
+    z.dz = g.dz;
     return z;
 }
 
 // Returns 1 when the launched kernel also filled the ghost cells of out.
 int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
                         const double* u, const double* v, const double* w, const double* sl, double dt,
-                        hipStream_t st) {
+                        hipStream_t st, bool exact) {
     const int kz = g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16);
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
         return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
     const ZGeom z = make_z(g, kz, FX, FY);
     if (z.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
     const dim3 grid((unsigned)z.nblocks), block(FX, FY, 1);
-    if (stage == 1)
-        hipLaunchKernelGGL((k_stage_zface<1>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
-    else if (stage == 2)
-        hipLaunchKernelGGL((k_stage_zface<2>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
-    else
-        hipLaunchKernelGGL((k_stage_zface<3>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+#define LS_ZFACE_LAUNCH(E)                                                                                      \
+    if (stage == 1)                                                                                             \
+        hipLaunchKernelGGL((k_stage_zface<1, E>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);        \
+    else if (stage == 2)                                                                                        \
+        hipLaunchKernelGGL((k_stage_zface<2, E>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);        \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_stage_zface<3, E>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+    if (exact) {
+        LS_ZFACE_LAUNCH(true)
+    } else {
+        LS_ZFACE_LAUNCH(false)
+    }
+#undef LS_ZFACE_LAUNCH
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "z-march stage launch failed");
     return z.fuse_ghosts ? 1 : 0;
+}
+
+int launch_stage_exact3d(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
+                         const double* u, const double* v, const double* w, const double* sl, double dt,
+                         hipStream_t st) {
+    return launch_stage_zmarch(g, stage, Gs, Gn, out, u, v, w, sl, dt, st, true);
 }
 
 template <int DIM>
 int launch_stage_fast(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
                       const double* u, const double* v, const double* w, const double* sl, double dt,
                       hipStream_t st) {
-    if (DIM == 3) return launch_stage_zmarch(g, stage, Gs, Gn, out, u, v, w, sl, dt, st);
+    if (DIM == 3) return launch_stage_zmarch(g, stage, Gs, Gn, out, u, v, w, sl, dt, st, false);
     const FastGeom f = make_fast(g);
     if (f.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
     const dim3 grid((unsigned)f.nblocks), block(kFX, kFY, 1);

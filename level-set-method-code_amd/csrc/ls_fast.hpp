@@ -9,6 +9,11 @@ template <int DIM>
 int launch_stage_fast(const lsx::Geom& g, int stage, const double* Gs, const double* Gn,
                       double* out, const double* u, const double* v, const double* w,
                       const double* sl, double dt, hipStream_t st);
+// Exact-mode 3D stage on the shared-face z-march (bitwise the CPU oracle).
+// Returns 1 when the kernel also wrote the ghost cells of out.
+int launch_stage_exact3d(const lsx::Geom& g, int stage, const double* Gs, const double* Gn, double* out,
+                         const double* u, const double* v, const double* w, const double* sl, double dt,
+                         hipStream_t st);
 int launch_hcr2_fast(const lsx::Geom& g, const double* pn, const double* p0, double* pnew,
                      hipStream_t st);
 int launch_hcr2_pair_fast(const lsx::Geom& g, const double* p0, double* pnew, hipStream_t st);

@@ -334,6 +334,7 @@ static int launch_stage(const ls_solver* s, int stage, const double* Gs, const d
                         const double* sl, double dt, hipStream_t st) {
     const Geom& g = s->g;
     if (s->p.mode == LS_MODE_FAST) return lsf::launch_stage_fast<DIM>(g, stage, Gs, Gn, out, u, v, w, sl, dt, st);
+    if constexpr (DIM == 3) return lsf::launch_stage_exact3d(g, stage, Gs, Gn, out, u, v, w, sl, dt, st);
     dim3 grid = cell_grid(g), block(kTX, kTY, 1);
     if (stage == 1)
         hipLaunchKernelGGL((k_stage_exact<DIM, 1>), grid, block, 0, st, g, Gs, Gn, out, u, v, w, sl, dt);
