@@ -39,8 +39,22 @@ struct FastGeom {
     double inv6dx, inv6dy, inv6dz;
     double invdx, invdy, invdz;
     int tiles_x, tiles_y;      // block tiles per plane
+    int fuse_ghosts;           // 2D stage kernel writes the ghost images of its output
     int64_t nblocks;
 };
+
+// The a-th padded index that holds interior index q after a ghost fill
+// (a = 0 is the cell itself), and how many there are.
+__device__ __forceinline__ int n_images(int q, int n, int bc) {
+    if (bc == LS_BC_PERIODIC) return 1 + (q < 3) + (q >= n - 3);
+    if (bc == LS_BC_ZERO_GRAD) return 1 + 3 * (q == 0) + 3 * (q == n - 1);
+    return 1;
+}
+__device__ __forceinline__ int image(int q, int n, int bc, int a) {
+    if (a == 0) return q + 3;
+    if (bc == LS_BC_PERIODIC) return (a == 1 && q < 3) ? q + 3 + n : q + 3 - n;
+    return q == 0 ? a - 1 : n + 2 + a;  // zero-gradient: 0,1,2 or n+3,n+4,n+5
+}
 
 constexpr int kFX = 64;
 constexpr int kFY = 4;
@@ -111,6 +125,16 @@ __global__ __launch_bounds__(256) void k_stage_fast(FastGeom f, const double* __
     else if (STAGE == 2) o = fma(0.25 * dt, rr, fma(0.25, Gs[c], 0.75 * gn));
     else o = fma((2.0 / 3.0) * dt, rr, fma(2.0 / 3.0, Gs[c], (1.0 / 3.0) * gn));
     out[c] = o;
+    // ghost cells of out: written by the interior cell they copy (periodic /
+    // zero-gradient), which replaces the separate ghost-fill launch
+    if (DIM == 2 && f.fuse_ghosts && (i < 3 || i >= f.g.nx - 3 || j < 3 || j >= f.g.ny - 3)) {
+        const int nxi = n_images(i, f.g.nx, f.g.bc[0]), nyi = n_images(j, f.g.ny, f.g.bc[1]);
+        for (int b = 0; b < nyi; b++) {
+            const int64_t pj = (int64_t)image(j, f.g.ny, f.g.bc[1], b) * f.g.sy;
+            for (int e = 0; e < nxi; e++)
+                if (b | e) out[pj + image(i, f.g.nx, f.g.bc[0], e)] = o;
+        }
+    }
 }
 
 // HCR-2 pseudo-time step (reinitialization.h:136-193) with multiplies for the
@@ -211,18 +235,6 @@ struct ZGeom {
     double dx, dy, dz;            // exact-mode stage kernel: the reference divides by these
 };
 
-// The a-th padded index that holds interior index q after a ghost fill
-// (a = 0 is the cell itself), and how many there are.
-__device__ __forceinline__ int n_images(int q, int n, int bc) {
-    if (bc == LS_BC_PERIODIC) return 1 + (q < 3) + (q >= n - 3);
-    if (bc == LS_BC_ZERO_GRAD) return 1 + 3 * (q == 0) + 3 * (q == n - 1);
-    return 1;
-}
-__device__ __forceinline__ int image(int q, int n, int bc, int a) {
-    if (a == 0) return q + 3;
-    if (bc == LS_BC_PERIODIC) return (a == 1 && q < 3) ? q + 3 + n : q + 3 - n;
-    return q == 0 ? a - 1 : n + 2 + a;  // zero-gradient: 0,1,2 or n+3,n+4,n+5
-}
 
 // ---------------------------------------------------------------------------
 // 3D stage kernel with shared faces.  The upwind derivative of a cell along
@@ -247,51 +259,6 @@ __device__ __forceinline__ int image(int q, int n, int bc, int a) {
 //    N_a = n_hi d_lo - n_lo d_hi,  D_a = d_hi d_lo.
 // Smoothness terms T(c) = (eps + beta)^2 of the sub-stencil centred at c do
 // not depend on the orientation and are computed once per point.
-struct Face {
-    double n, d;
-};
-
-// Face between cells c and c+1.  v0..v4 run from the far upwind side: G(c-2..c+2)
-// left-biased, G(c+3..c-1) mirrored; t0..t2 are T at the centres of v1..v3.
-// The linear weights (0.1, 0.6, 0.3) / (0.3, 0.6, 0.1) are scaled by 10,
-// a factor common to numerator and denominator.
-template <bool MIRROR>
-__device__ __forceinline__ Face face_core(double v0, double v1, double v2, double v3, double v4, double t0,
-                                          double t1, double t2) {
-    const double p0 = fma(2.0, v0, fma(-7.0, v1, 11.0 * v2));
-    const double p1 = fma(5.0, v2, fma(2.0, v3, -v1));
-    const double p2 = fma(2.0, v2, fma(5.0, v3, -v4));
-    const double w0 = MIRROR ? 3.0 * (t1 * t2) : t1 * t2;
-    const double w1 = 6.0 * (t0 * t2);
-    const double w2 = MIRROR ? t0 * t1 : 3.0 * (t0 * t1);
-    Face f;
-    f.n = fma(w0, p0, fma(w1, p1, w2 * p2));
-    f.d = (w0 + w1) + w2;
-    return f;
-}
-
-// The face c+1/2 from LDS: g points at G(c), t at T(c); GS and TS are the
-// strides of the axis in g and t.
-template <bool MIRROR, int GS, int TS>
-__device__ __forceinline__ Face face_lds(const double* g, const double* t) {
-    if (MIRROR) return face_core<true>(g[3 * GS], g[2 * GS], g[GS], g[0], g[-GS], t[2 * TS], t[TS], t[0]);
-    return face_core<false>(g[-2 * GS], g[-GS], g[0], g[GS], g[2 * GS], t[-TS], t[0], t[TS]);
-}
-
-// Stores the face c+1/2 in each orientation asked for: slot[0] left-biased,
-// slot[OS] mirrored.
-template <int GS, int TS, int OS>
-__device__ __forceinline__ void face_put(const double* g, const double* t, bool needL, bool needR, double2* slot) {
-    if (needL) {
-        const Face f = face_lds<false, GS, TS>(g, t);
-        slot[0] = make_double2(f.n, f.d);
-    }
-    if (needR) {
-        const Face f = face_lds<true, GS, TS>(g, t);
-        slot[OS] = make_double2(f.n, f.d);
-    }
-}
-
 // Ghost images of a freshly computed interior value (periodic / zero-gradient).
 __device__ __forceinline__ void store_images(const ZGeom& z, double* __restrict__ out, int i, int j, int k,
                                              double o) {
@@ -880,6 +847,8 @@ static FastGeom make_fast(const Geom& g) {
     f.invdz = 1.0 / g.dz;
     f.tiles_x = (g.nx + kFX - 1) / kFX;
     f.tiles_y = (g.ny + kFY - 1) / kFY;
+    f.fuse_ghosts = g.bc[0] != LS_BC_LIN_EXTRAP && g.bc[1] != LS_BC_LIN_EXTRAP &&
+                    (g.dim == 2 || g.bc[2] != LS_BC_LIN_EXTRAP);
     const int64_t tiles = (int64_t)f.tiles_x * f.tiles_y * (g.dim == 3 ? g.nz : 1);
     f.nblocks = ((tiles + 7) / 8) * 8;
     return f;
@@ -963,7 +932,7 @@ int launch_stage_fast(const Geom& g, int stage, const double* Gs, const double* 
     else
         hipLaunchKernelGGL((k_stage_fast<DIM, 3>), grid, block, 0, st, f, Gs, Gn, out, u, v, w, sl, dt);
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast stage launch failed");
-    return LS_OK;
+    return f.fuse_ghosts ? 1 : 0;
 }
 
 template int launch_stage_fast<2>(const Geom&, int, const double*, const double*, double*, const double*,

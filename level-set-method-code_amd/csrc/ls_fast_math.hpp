@@ -99,4 +99,52 @@ __device__ __forceinline__ double dweno_t(double m3, double m2, double m1, doubl
     return div_fast(num, Da * Db) * (sg * inv6h);
 }
 
+// ---------------------------------------------------------------------------
+// Shared faces (k_stage_zface and the 2D plane kernel).  A face is kept as a
+// numerator n and denominator d, value n / (6 d).
+struct Face {
+    double n, d;
+};
+
+// Face between cells c and c+1.  v0..v4 run from the far upwind side: G(c-2..c+2)
+// left-biased, G(c+3..c-1) mirrored; t0..t2 are T at the centres of v1..v3.
+// The linear weights (0.1, 0.6, 0.3) / (0.3, 0.6, 0.1) are scaled by 10,
+// a factor common to numerator and denominator.
+template <bool MIRROR>
+__device__ __forceinline__ Face face_core(double v0, double v1, double v2, double v3, double v4, double t0,
+                                          double t1, double t2) {
+    const double p0 = fma(2.0, v0, fma(-7.0, v1, 11.0 * v2));
+    const double p1 = fma(5.0, v2, fma(2.0, v3, -v1));
+    const double p2 = fma(2.0, v2, fma(5.0, v3, -v4));
+    const double w0 = MIRROR ? 3.0 * (t1 * t2) : t1 * t2;
+    const double w1 = 6.0 * (t0 * t2);
+    const double w2 = MIRROR ? t0 * t1 : 3.0 * (t0 * t1);
+    Face f;
+    f.n = fma(w0, p0, fma(w1, p1, w2 * p2));
+    f.d = (w0 + w1) + w2;
+    return f;
+}
+
+// The face c+1/2 from LDS: g points at G(c), t at T(c); GS and TS are the
+// strides of the axis in g and t.
+template <bool MIRROR, int GS, int TS>
+__device__ __forceinline__ Face face_lds(const double* g, const double* t) {
+    if (MIRROR) return face_core<true>(g[3 * GS], g[2 * GS], g[GS], g[0], g[-GS], t[2 * TS], t[TS], t[0]);
+    return face_core<false>(g[-2 * GS], g[-GS], g[0], g[GS], g[2 * GS], t[-TS], t[0], t[TS]);
+}
+
+// Stores the face c+1/2 in each orientation asked for: slot[0] left-biased,
+// slot[OS] mirrored.
+template <int GS, int TS, int OS>
+__device__ __forceinline__ void face_put(const double* g, const double* t, bool needL, bool needR, double2* slot) {
+    if (needL) {
+        const Face f = face_lds<false, GS, TS>(g, t);
+        slot[0] = make_double2(f.n, f.d);
+    }
+    if (needR) {
+        const Face f = face_lds<true, GS, TS>(g, t);
+        slot[OS] = make_double2(f.n, f.d);
+    }
+}
+
 }  // namespace lsf
