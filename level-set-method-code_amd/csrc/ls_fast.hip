@@ -221,6 +221,18 @@ constexpr int ZX = 64, ZY = 4, ZH = 3;
 #ifndef LS_ZMARCH_KZ
 #define LS_ZMARCH_KZ 32  // z planes per block at large nz
 #endif
+// Planes per block for a z-march over tiles of tile_cols x tile_rows cells:
+// LS_ZMARCH_KZ where the grid is large, shorter chunks (down to 8 planes)
+// where that keeps at least ~2048 blocks in flight (two rounds of four
+// blocks on each of 256 CUs), since a chunk's 6-plane start-up is cheaper
+// than idle CUs.
+static int zmarch_chunk(const Geom& g, int tile_cols, int tile_rows) {
+    const int64_t tiles = (int64_t)((g.nx + tile_cols - 1) / tile_cols) * ((g.ny + tile_rows - 1) / tile_rows);
+    const int64_t want = 2048;
+    int kz = LS_ZMARCH_KZ;
+    while (kz > 8 && tiles * ((g.nz + kz - 1) / kz) < want) kz /= 2;
+    return kz;
+}
 
 // Kernel arguments of the z-marching kernels, kept small so that they stay in
 // scalar registers.
@@ -888,7 +900,7 @@ static ZGeom make_z(const Geom& g, int kz, int tile_cols = ZX, int tile_rows = Z
 int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
                         const double* u, const double* v, const double* w, const double* sl, double dt,
                         hipStream_t st, bool exact) {
-    const int kz = g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16);
+    const int kz = zmarch_chunk(g, FX, FY);
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
         return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
     const ZGeom z = make_z(g, kz, FX, FY);
@@ -948,7 +960,7 @@ int launch_hcr2_pair_fast(const Geom& g, const double* p0, double* pnew, hipStre
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
         return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
     HGeom hg;
-    hg.z = make_z(g, g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16));
+    hg.z = make_z(g, zmarch_chunk(g, HX, HY));
     if (!hg.z.fuse_ghosts) return 0;
     hg.dx2 = g.dx * g.dx;
     hg.invdx = 1.0 / g.dx;
@@ -968,7 +980,7 @@ int launch_hcr2_fast(const Geom& g, const double* pn, const double* p0, double* 
         if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
             return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
         HGeom hg;
-        hg.z = make_z(g, g.nz >= 4 * LS_ZMARCH_KZ ? LS_ZMARCH_KZ : (g.nz >= 64 ? 32 : 16));
+        hg.z = make_z(g, zmarch_chunk(g, HX, HY));
         hg.dx2 = g.dx * g.dx;
         hg.invdx = 1.0 / g.dx;
         hg.invdy = 1.0 / g.dy;
