@@ -36,6 +36,7 @@
 
 #include "ls_api.h"
 #include "ls_exact.hpp"
+#include "ls_fast_math.hpp"
 #include "ls_internal.hpp"
 
 namespace {
@@ -66,24 +67,29 @@ __device__ __forceinline__ double beta_g(double a, double b, double c, double& d
 
 // Left-biased WENO-5 face of (v0..v4) with linear weights (d0,d1,d2), and its
 // gradient df[0..4].  The right-biased face of the reference is this on the
-// reversed stencil with weights (0.3, 0.6, 0.1) (weno5.h:60-81).
+// reversed stencil with weights (0.3, 0.6, 0.1) (weno5.h:60-81).  Divisions
+// are refined reciprocals (one per smoothness term, one for the weight sum):
+// the adjoint is a derivative, held to the tangent checker's tolerance, not to
+// the forward's bits.
 __device__ __forceinline__ double face_grad(const double* v, double d0, double d1, double d2, double* df) {
     double gb[3][3], b[3];
     b[0] = beta_g(v[0], v[1], v[2], gb[0][0], gb[0][1], gb[0][2]);
     b[1] = beta_g(v[1], v[2], v[3], gb[1][0], gb[1][1], gb[1][2]);
     b[2] = beta_g(v[2], v[3], v[4], gb[2][0], gb[2][1], gb[2][2]);
     const double d[3] = {d0, d1, d2};
-    double a[3], s = 0.0;
+    double a[3], re[3], s = 0.0;
     for (int k = 0; k < 3; k++) {
-        const double e = lsx::kWenoEps + b[k];
-        a[k] = d[k] / (e * e);
+        re[k] = lsf::rcp_nr(lsx::kWenoEps + b[k]);
+        a[k] = d[k] * (re[k] * re[k]);
         s += a[k];
     }
-    const double p[3] = {(2.0 * v[0] - 7.0 * v[1] + 11.0 * v[2]) / 6.0, (-v[1] + 5.0 * v[2] + 2.0 * v[3]) / 6.0,
-                         (2.0 * v[2] + 5.0 * v[3] - v[4]) / 6.0};
+    constexpr double s6 = 1.0 / 6.0;
+    const double p[3] = {(2.0 * v[0] - 7.0 * v[1] + 11.0 * v[2]) * s6, (-v[1] + 5.0 * v[2] + 2.0 * v[3]) * s6,
+                         (2.0 * v[2] + 5.0 * v[3] - v[4]) * s6};
+    const double rs = lsf::rcp_nr(s);
     double w[3], f = 0.0;
     for (int k = 0; k < 3; k++) {
-        w[k] = a[k] / s;
+        w[k] = a[k] * rs;
         f += w[k] * p[k];
     }
     df[0] = w[0] * (2.0 / 6.0);
@@ -93,7 +99,7 @@ __device__ __forceinline__ double face_grad(const double* v, double d0, double d
     df[4] = w[2] * (-1.0 / 6.0);
     for (int k = 0; k < 3; k++) {
         // d f / d beta_k = -2 w_k (p_k - f) / (eps + beta_k)
-        const double gk = -2.0 * w[k] * (p[k] - f) / (lsx::kWenoEps + b[k]);
+        const double gk = -2.0 * w[k] * (p[k] - f) * re[k];
         df[k] += gk * gb[k][0];
         df[k + 1] += gk * gb[k][1];
         df[k + 2] += gk * gb[k][2];
@@ -102,16 +108,16 @@ __device__ __forceinline__ double face_grad(const double* v, double d0, double d
 }
 
 // Upwind WENO-5 derivative along one axis from the stencil s7 = (m3..p3), and
-// its gradient P[0..6].
-__device__ __forceinline__ double dweno_grad(const double* s7, double vel, double h, double* P) {
+// its gradient P[0..6]; ih = 1/h.
+__device__ __forceinline__ double dweno_grad(const double* s7, double vel, double ih, double* P) {
     double fa, fb, ga[5], gb[5];
     for (int q = 0; q < 7; q++) P[q] = 0.0;
     if (vel >= 0.0) {
         fa = face_grad(s7 + 0, 0.1, 0.6, 0.3, ga);  // face i-1/2 from m3..p1
         fb = face_grad(s7 + 1, 0.1, 0.6, 0.3, gb);  // face i+1/2 from m2..p2
         for (int q = 0; q < 5; q++) {
-            P[q] -= ga[q] / h;
-            P[q + 1] += gb[q] / h;
+            P[q] -= ga[q] * ih;
+            P[q + 1] += gb[q] * ih;
         }
     } else {
         const double ra[5] = {s7[5], s7[4], s7[3], s7[2], s7[1]};  // face i-1/2, mirrored m2..p2
@@ -119,11 +125,11 @@ __device__ __forceinline__ double dweno_grad(const double* s7, double vel, doubl
         fa = face_grad(ra, 0.3, 0.6, 0.1, ga);
         fb = face_grad(rb, 0.3, 0.6, 0.1, gb);
         for (int q = 0; q < 5; q++) {
-            P[5 - q] -= ga[q] / h;
-            P[6 - q] += gb[q] / h;
+            P[5 - q] -= ga[q] * ih;
+            P[6 - q] += gb[q] * ih;
         }
     }
-    return (fb - fa) / h;
+    return (fb - fa) * ih;
 }
 
 // Plane offsets.  19-point stencil: 0 centre, 1..6 x -3,-2,-1,+1,+2,+3,
@@ -162,22 +168,24 @@ __global__ __launch_bounds__(256) void k_adj_stage(AGeom g, const double* __rest
     const double lam = cdt * lam_out[c];
     const double* G = Gs + c;
     const double s_l = sl ? sl[c] : sl0;
-    double gx = 0.0, gy = 0.0, gz = 0.0, m = 1.0;
+    const double ihx = 1.0 / g.dx, ihy = 1.0 / g.dy, ihz = 1.0 / g.dz;
+    double gx = 0.0, gy = 0.0, gz = 0.0, m = 1.0, im = 1.0;
     const bool on = s_l > g.eps;
-    if (on) {
+    if (on) {  // u_eff in the exact forward's arithmetic: its sign picks the upwind side
         gx = (G[1] - G[-1]) / (2.0 * g.dx);
         gy = (G[g.sy] - G[-g.sy]) / (2.0 * g.dy);
         gz = (G[g.sz] - G[-g.sz]) / (2.0 * g.dz);
         m = sqrt(gx * gx + gy * gy + gz * gz + g.eps);
+        im = lsf::rcp_nr(m);
     }
     const double ue = u[c] - s_l * gx / m, ve = v[c] - s_l * gy / m, we = w[c] - s_l * gz / m;
     double s7[7], Px[7], Py[7], Pz[7];
     for (int q = 0; q < 7; q++) s7[q] = G[q - 3];
-    const double dgx = dweno_grad(s7, ue, g.dx, Px);
+    const double dgx = dweno_grad(s7, ue, ihx, Px);
     for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sy];
-    const double dgy = dweno_grad(s7, ve, g.dy, Py);
+    const double dgy = dweno_grad(s7, ve, ihy, Py);
     for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sz];
-    const double dgz = dweno_grad(s7, we, g.dz, Pz);
+    const double dgz = dweno_grad(s7, we, ihz, Pz);
     double out[19];
     out[0] = -(ue * Px[3] + ve * Py[3] + we * Pz[3]);
     for (int q = 0; q < 6; q++) {
@@ -189,16 +197,17 @@ __global__ __launch_bounds__(256) void k_adj_stage(AGeom g, const double* __rest
     if (on) {
         // d RHS / d g_b = s_l (dg_b - g_b Q / m^2) / m with Q = sum_a dg_a g_a
         const double Q = dgx * gx + dgy * gy + dgz * gz;
-        const double rx = s_l * (dgx - gx * Q / (m * m)) / m;
-        const double ry = s_l * (dgy - gy * Q / (m * m)) / m;
-        const double rz = s_l * (dgz - gz * Q / (m * m)) / m;
-        out[3] -= rx / (2.0 * g.dx);
-        out[4] += rx / (2.0 * g.dx);
-        out[9] -= ry / (2.0 * g.dy);
-        out[10] += ry / (2.0 * g.dy);
-        out[15] -= rz / (2.0 * g.dz);
-        out[16] += rz / (2.0 * g.dz);
-        if (gsl) gsl[c] += lam * (Q / m);
+        const double qm = Q * (im * im);
+        const double rx = s_l * (dgx - gx * qm) * im;
+        const double ry = s_l * (dgy - gy * qm) * im;
+        const double rz = s_l * (dgz - gz * qm) * im;
+        out[3] -= rx * (0.5 * ihx);
+        out[4] += rx * (0.5 * ihx);
+        out[9] -= ry * (0.5 * ihy);
+        out[10] += ry * (0.5 * ihy);
+        out[15] -= rz * (0.5 * ihz);
+        out[16] += rz * (0.5 * ihz);
+        if (gsl) gsl[c] += lam * (Q * im);
     }
     for (int q = 0; q < 19; q++) P[(int64_t)q * g.n + c] = lam * out[q];
     gu[c] -= lam * dgx;
