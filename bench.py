@@ -14,10 +14,13 @@ Prints ONE JSON line (rank 0).  Options:
   --no-cpu-baseline        skip the CPU reference timing
   --no-adjoint             skip the reduced adjoint-vs-FD leg that the default
                            line carries under "adjoint" (4 parameter sets)
+  --no-extra               skip the "other_mode" (the other arithmetic mode,
+                           10 steps of the same workload) and "batch" (the
+                           batched ensemble below) summaries of the default line
   --profile-pass           launch a 1 GiB copy, a ghost fill and stage kernels
                            only (for rocprofv3 --pmc runs)
   --summarize-pmc F W      turn two rocprofv3 --pmc CSV directories (FETCH_SIZE,
-                           WRITE_SIZE) into profiles/stage_kernel_traffic_512.json
+                           WRITE_SIZE) into profiles/stage_kernel_traffic_512_<mode>.json
   --config batch           the batched-ensemble configuration instead: 96 2D
                            flames of seeded sizes in {128,192,256,384,512}^2,
                            S_L ~ U[0.05,0.2], 300 RK3 steps each, in one
@@ -48,7 +51,9 @@ METRIC = "cell-updates/s (WENO5-RK3+HCR2 512³ FP64), % HBM peak; adjoint vs FD 
 HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
 STAGE_BYTES = {1: 40, 2: 48, 3: 48}   # algorithmic bytes per interior cell per stage
 STEP_BYTES = 136 + 83 / 5.0           # RK3 + amortised HCR-2, bytes per cell-update
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_kernel_traffic_512.json")
+def traffic_file(mode):
+    """Measured HBM bytes of the stage-2 kernel at 512^3, one file per arithmetic mode."""
+    return os.path.join(ROOT, "profiles", "stage_kernel_traffic_512_%s.json" % mode)
 # the 3D stage kernels (shared-face z-march), fast and exact arithmetic
 FAST_STAGE_KERNEL = "k_stage_zface<2, false>"
 EXACT_STAGE_KERNEL = "k_stage_zface<2, true>"
@@ -163,7 +168,7 @@ def ensemble(ls, count=96, seed=4, mode="fast"):
     return members, params, fields
 
 
-def bench_batch(ls, steps, mode):
+def bench_batch(ls, steps, mode, emit=True):
     import torch
     members, params, fields = ensemble(ls, mode=mode)
     b = ls.Batch(members, mode=ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT)
@@ -210,7 +215,9 @@ def bench_batch(ls, steps, mode):
                       "autotune_probe_ms_per_step": probe_ms},
            "batched_s": t_batch, "serial_loop_s": t_serial, "speedup_vs_serial": t_serial / t_batch,
            "serial_cell_updates_per_s": cells * steps / t_serial, "bitwise_equal_to_serial": bool(same)}
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_adjoint(ls, mode, nsets, fd_sample=16, emit=True):
@@ -336,10 +343,11 @@ def bench_sample(ls, steps, n=800):
 
 
 def read_traffic(n, mode):
-    if n != 512 or not os.path.exists(TRAFFIC_FILE):
+    path = traffic_file(mode)
+    if n != 512 or not os.path.exists(path):
         return None
     try:
-        d = json.load(open(TRAFFIC_FILE))
+        d = json.load(open(path))
         if d.get("mode") != mode:
             return None
         return d.get("hbm_bytes_per_launch")
@@ -373,7 +381,7 @@ def _pmc_rows(d):
     return rows
 
 
-def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=TRAFFIC_FILE):
+def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=None):
     """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) for the stage-2 kernel, each
     scaled by what the same counter reads for the 1 GiB copy kernel."""
     def per_kernel(rows, counter):
@@ -404,10 +412,50 @@ def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=TRAFFIC_FILE):
          "hbm_bytes_per_launch": fetch + write,
          "hbm_bytes_per_cell": (fetch + write) / cells,
          "algorithmic_bytes_per_cell": STAGE_BYTES[2]}
+    out_path = out_path or traffic_file(mode)
     os.makedirs(os.path.dirname(os.path.abspath(out_path)), exist_ok=True)
     with open(out_path, "w") as fh:
         json.dump(d, fh, indent=1)
     print(json.dumps(d))
+
+
+def timed_steps(ls, n, mode, steps, warmup):
+    """Whole-job rate of `steps` solver steps on the n^3 workload, plus the stage kernel's roofline."""
+    import torch
+    p, s, G, u, v, w, dt = setup(ls, n, mode)
+    cells = p.cells
+    # warmup (untimed); steps counted from 1 so that HCR-2 runs exactly once per 5 steps
+    s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=1, max_steps=warmup)
+    torch.cuda.synchronize()
+    step0 = 1 + warmup
+    st = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(st)
+    n_done, _, _ = s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=step0, max_steps=steps)
+    ev1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev = ev0.elapsed_time(ev1) * 1e-3
+    assert n_done == steps
+    elapsed = max(wall, dev)
+    value = cells * steps / elapsed
+    ok = s.check_finite(G) == 0
+
+    t_stage, t_fill = time_kernels(ls, s, G, u, v, w, dt, fused=(mode == "fast"))
+    ach = STAGE_BYTES[2] * cells / t_stage
+    traffic = read_traffic(n, mode)
+    roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": ach / HBM_PEAK, "traffic": traffic,
+            "kernel": (FAST_STAGE_KERNEL if mode == "fast" else EXACT_STAGE_KERNEL) +
+                      " (fused WENO5/u_eff RHS + stage-2 combine)",
+            "kernel_ms": t_stage * 1e3, "ghost_fill_ms": t_fill * 1e3,
+            "algorithmic_bytes_per_cell": STAGE_BYTES[2],
+            "step_frac": value * STEP_BYTES / HBM_PEAK}
+    del s, G, u, v, w
+    torch.cuda.empty_cache()
+    return value, elapsed, roof, dt, ok
 
 
 def main():
@@ -420,10 +468,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-pass", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
-    ap.add_argument("--traffic-out", default=TRAFFIC_FILE)
+    ap.add_argument("--traffic-out", default=None)
     ap.add_argument("--config", choices=["c3", "batch", "adjoint", "sample"], default="c3")
     ap.add_argument("--sets", type=int, default=32)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other-mode and batched-ensemble summaries on the default line")
     a = ap.parse_args()
 
     if a.summarize_pmc:
@@ -451,37 +501,7 @@ def main():
         bench_sample(ls, 2000 if a.steps == 20 else a.steps)
         return
 
-    p, s, G, u, v, w, dt = setup(ls, a.n, a.mode)
-    cells = p.cells
-    # warmup (untimed); steps counted from 1 so that HCR-2 runs exactly once per 5 steps
-    s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=1, max_steps=a.warmup)
-    torch.cuda.synchronize()
-    step0 = 1 + a.warmup
-    st = torch.cuda.current_stream()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(st)
-    n_done, _, _ = s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=step0, max_steps=a.steps)
-    ev1.record(st)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    dev = ev0.elapsed_time(ev1) * 1e-3
-    assert n_done == a.steps
-    elapsed = max(wall, dev)
-    value = cells * a.steps / elapsed
-    ok = s.check_finite(G) == 0
-
-    t_stage, t_fill = time_kernels(ls, s, G, u, v, w, dt, fused=(a.mode == "fast"))
-    ach = STAGE_BYTES[2] * cells / t_stage
-    traffic = read_traffic(a.n, a.mode)
-    roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": ach / HBM_PEAK, "traffic": traffic,
-            "kernel": (FAST_STAGE_KERNEL if a.mode == "fast" else EXACT_STAGE_KERNEL) +
-                      " (fused WENO5/u_eff RHS + stage-2 combine)",
-            "kernel_ms": t_stage * 1e3, "ghost_fill_ms": t_fill * 1e3,
-            "algorithmic_bytes_per_cell": STAGE_BYTES[2],
-            "step_frac": value * STEP_BYTES / HBM_PEAK}
+    value, elapsed, roof, dt, ok = timed_steps(ls, a.n, a.mode, a.steps, a.warmup)
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -500,6 +520,19 @@ def main():
                                                "fd_s_per_gradient", "speedup_vs_fd", "fd_sample")}
         out["adjoint"]["workload"] = adj["config"]["workload"]
         out["adjoint"]["parameter_sets"] = 4
+    if not a.no_extra:
+        # the other arithmetic mode on the same workload (short run), and the batched 2D ensemble
+        other = "exact" if a.mode == "fast" else "fast"
+        ov, oe, oroof, _, ook = timed_steps(ls, a.n, other, 10, 2)
+        out["other_mode"] = {"mode": other, "value": ov, "unit": "cell-updates/s", "steps": 10,
+                             "ms_per_step": oe / 10 * 1e3, "kernel": oroof["kernel"],
+                             "kernel_ms": oroof["kernel_ms"], "frac": oroof["frac"],
+                             "traffic": oroof["traffic"], "finite": ook}
+        bt = bench_batch(ls, 300, a.mode, emit=False)
+        out["batch"] = {k: bt[k] for k in ("value", "unit", "steps", "batched_s", "serial_loop_s",
+                                           "speedup_vs_serial", "bitwise_equal_to_serial")}
+        out["batch"]["workload"] = bt["config"]["workload"]
+        out["batch"]["tile_shape"] = bt["config"]["tile_shape"]
     print(json.dumps(out), flush=True)
 
 
