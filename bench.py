@@ -179,7 +179,7 @@ def bench_batch(ls, steps, mode, emit=True):
         b.view(V, q).copy_(v)
     b.fill_ghosts(G)
     G0 = G.clone()
-    shape, probe_ms = b.autotune(G, U, V, probe_steps=3)
+    shape, probe_ms = b.autotune(G, U, V, probe_steps=10)
     b.advance(G, U, V, steps, graph=True)  # capture + first replay (warm-up)
     torch.cuda.synchronize()
     G.copy_(G0)

@@ -350,7 +350,7 @@ class Batch:
     def tile(self):
         return lib().ls_batch_get_tile(self.h)
 
-    def autotune(self, G, u, v, probe_steps=3, stream=None):
+    def autotune(self, G, u, v, probe_steps=10, stream=None):
         ms = C.c_double()
         check(lib().ls_batch_autotune(self.h, _ptr(G), _ptr(u), _ptr(v), probe_steps, C.byref(ms),
                                       _stream(stream)))

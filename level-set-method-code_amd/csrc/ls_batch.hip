@@ -27,6 +27,7 @@
 #include "ls_exact.hpp"
 #include "ls_fast_math.hpp"
 #include "ls_internal.hpp"
+#include "ls_ymarch.hpp"
 
 namespace {
 
@@ -45,27 +46,22 @@ struct BTile {
 struct TileShape {
     int tx, ty;
 };
+// Exact mode: one thread per cell, tiles of tx x ty threads.  Fast mode: the
+// y-march (ls_ymarch.hpp), one wavefront per 64-column strip and chunk of
+// kChunks[shape] rows.
 constexpr TileShape kShapes[] = {{64, 4}, {64, 2}, {128, 2}, {32, 8}};
+constexpr int kChunks[] = {32, 16, 4, 8};
 constexpr int kNumShapes = 4;
 
-__device__ __forceinline__ int b_nimg(int q, int n, int bc) {
-    if (bc == LS_BC_PERIODIC) return 1 + (q < 3) + (q >= n - 3);
-    return 1 + 3 * (q == 0) + 3 * (q == n - 1);
-}
-__device__ __forceinline__ int b_img(int q, int n, int bc, int a) {
-    if (a == 0) return q + 3;
-    if (bc == LS_BC_PERIODIC) return (a == 1 && q < 3) ? q + 3 + n : q + 3 - n;
-    return q == 0 ? a - 1 : n + 2 + a;
-}
-
-template <int MODE, int STAGE, int TX, int TY>
+template <int STAGE, int TX, int TY>
 __global__ __launch_bounds__(TX* TY) void k_batch_stage(const BMember* __restrict__ mem,
-                                                        const BTile* __restrict__ tiles, int ntiles,
+                                                        const BTile* __restrict__ tiles, int ntiles, int ch,
                                                         const double* __restrict__ Gs,
                                                         const double* __restrict__ Gn,
                                                         double* __restrict__ out,
                                                         const double* __restrict__ u,
                                                         const double* __restrict__ v) {
+    (void)ch;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const BTile tl = tiles[t];
         const BMember m = mem[tl.member];
@@ -75,47 +71,62 @@ __global__ __launch_bounds__(TX* TY) void k_batch_stage(const BMember* __restric
         const int64_t sy = m.nxt;
         const int64_t c = m.off + (int64_t)(j + 3) * sy + (i + 3);
         const double* P = Gs + c;
-        double r;
-        if (MODE == LS_MODE_EXACT) {
-            lsx::Geom g;
-            g.sy = sy;
-            g.sz = 0;
-            g.dx = m.dx;
-            g.dy = m.dy;
-            g.eps = m.eps;
-            r = lsx::rhs_cell<2>(g, P, u[c], v[c], 0.0, m.s_l);
-        } else {
-            double ue = u[c], ve = v[c];
-            if (m.s_l > m.eps) {
-                const double gx = (P[1] - P[-1]) * m.inv2dx;
-                const double gy = (P[sy] - P[-sy]) * m.inv2dy;
-                const double k = m.s_l * lsf::rsqrt_nr(fma(gx, gx, gy * gy) + m.eps);
-                ue = fma(-k, gx, ue);
-                ve = fma(-k, gy, ve);
-            }
-            const double dgx = lsf::dweno_fast(P[-3], P[-2], P[-1], P[0], P[1], P[2], P[3], ue, m.inv6dx);
-            const double dgy = lsf::dweno_fast(P[-3 * sy], P[-2 * sy], P[-sy], P[0], P[sy], P[2 * sy], P[3 * sy],
-                                               ve, m.inv6dy);
-            r = -fma(ue, dgx, ve * dgy);
-        }
-        double o;
-        if (MODE == LS_MODE_EXACT) {
-            o = lsx::stage_combine<STAGE>(Gn[c], P[0], r, m.dt);
-        } else {
-            const double gn = Gn[c];
-            if (STAGE == 1) o = fma(m.dt, r, gn);
-            else if (STAGE == 2) o = fma(0.25 * m.dt, r, fma(0.25, P[0], 0.75 * gn));
-            else o = fma((2.0 / 3.0) * m.dt, r, fma(2.0 / 3.0, P[0], (1.0 / 3.0) * gn));
-        }
+        lsx::Geom g;
+        g.sy = sy;
+        g.sz = 0;
+        g.dx = m.dx;
+        g.dy = m.dy;
+        g.eps = m.eps;
+        const double r = lsx::rhs_cell<2>(g, P, u[c], v[c], 0.0, m.s_l);
+        const double o = lsx::stage_combine<STAGE>(Gn[c], P[0], r, m.dt);
         out[c] = o;
         if (i < 3 || i >= m.nx - 3 || j < 3 || j >= m.ny - 3) {
-            const int nxi = b_nimg(i, m.nx, m.bc0), nyi = b_nimg(j, m.ny, m.bc1);
+            const int nxi = lsf::n_images(i, m.nx, m.bc0), nyi = lsf::n_images(j, m.ny, m.bc1);
             for (int b = 0; b < nyi; b++) {
-                const int64_t pj = m.off + (int64_t)b_img(j, m.ny, m.bc1, b) * sy;
+                const int64_t pj = m.off + (int64_t)lsf::image(j, m.ny, m.bc1, b) * sy;
                 for (int e = 0; e < nxi; e++)
-                    if (b | e) out[pj + b_img(i, m.nx, m.bc0, e)] = o;
+                    if (b | e) out[pj + lsf::image(i, m.nx, m.bc0, e)] = o;
             }
         }
+    }
+}
+
+// Fast mode: units (member, strip, chunk of ch rows), one wavefront each.
+template <int STAGE>
+__global__ __launch_bounds__(lsf::kYW) void k_batch_ymarch(const BMember* __restrict__ mem,
+                                                           const BTile* __restrict__ tiles, int ntiles, int ch,
+                                                           const double* __restrict__ Gs,
+                                                           const double* __restrict__ Gn,
+                                                           double* __restrict__ out,
+                                                           const double* __restrict__ u,
+                                                           const double* __restrict__ v) {
+    __shared__ lsf::YLds L;
+    // one unit per block, dealt XCD-contiguously (block b runs on XCD b % 8):
+    // XCD e takes units [e*per, (e+1)*per), so the consecutive chunks of a
+    // strip and its neighbour strips go through one L2
+    const int per = (int)(gridDim.x >> 3);
+    const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (t < ntiles) {
+        const BTile tl = tiles[t];
+        const BMember b = mem[tl.member];
+        lsf::YGrid m;
+        m.nx = b.nx;
+        m.ny = b.ny;
+        m.bc0 = b.bc0;
+        m.bc1 = b.bc1;
+        m.fuse_ghosts = 1;
+        m.pad = 0;
+        m.sy = b.nxt;
+        m.off = b.off;
+        m.s_l = b.s_l;
+        m.eps = b.eps;
+        m.dt = b.dt;
+        m.inv2dx = b.inv2dx;
+        m.inv2dy = b.inv2dy;
+        m.inv6dx = b.inv6dx;
+        m.inv6dy = b.inv6dy;
+        const int y0 = tl.ty * ch;
+        lsf::ymarch_strip<STAGE>(m, tl.tx * lsf::kYW, y0, min(y0 + ch, b.ny), Gs, Gn, out, u, v, nullptr, L);
     }
 }
 
@@ -141,31 +152,28 @@ __global__ __launch_bounds__(256) void k_copy(const double* __restrict__ a, doub
         b[t] = a[t];
 }
 
-typedef void (*StageFn)(const BMember*, const BTile*, int, const double*, const double*, double*, const double*,
-                        const double*);
+typedef void (*StageFn)(const BMember*, const BTile*, int, int, const double*, const double*, double*,
+                        const double*, const double*);
 
-template <int MODE, int TX, int TY>
+template <int TX, int TY>
 void pick3(StageFn* f) {
-    f[0] = k_batch_stage<MODE, 1, TX, TY>;
-    f[1] = k_batch_stage<MODE, 2, TX, TY>;
-    f[2] = k_batch_stage<MODE, 3, TX, TY>;
+    f[0] = k_batch_stage<1, TX, TY>;
+    f[1] = k_batch_stage<2, TX, TY>;
+    f[2] = k_batch_stage<3, TX, TY>;
 }
 
 void stage_fns(int mode, int shape, StageFn* f) {
-    if (mode == LS_MODE_EXACT) {
-        switch (shape) {
-            case 0: pick3<LS_MODE_EXACT, 64, 4>(f); break;
-            case 1: pick3<LS_MODE_EXACT, 64, 2>(f); break;
-            case 2: pick3<LS_MODE_EXACT, 128, 2>(f); break;
-            default: pick3<LS_MODE_EXACT, 32, 8>(f); break;
-        }
-    } else {
-        switch (shape) {
-            case 0: pick3<LS_MODE_FAST, 64, 4>(f); break;
-            case 1: pick3<LS_MODE_FAST, 64, 2>(f); break;
-            case 2: pick3<LS_MODE_FAST, 128, 2>(f); break;
-            default: pick3<LS_MODE_FAST, 32, 8>(f); break;
-        }
+    if (mode == LS_MODE_FAST) {
+        f[0] = k_batch_ymarch<1>;
+        f[1] = k_batch_ymarch<2>;
+        f[2] = k_batch_ymarch<3>;
+        return;
+    }
+    switch (shape) {
+        case 0: pick3<64, 4>(f); break;
+        case 1: pick3<64, 2>(f); break;
+        case 2: pick3<128, 2>(f); break;
+        default: pick3<32, 8>(f); break;
     }
 }
 
@@ -203,17 +211,20 @@ static void drop_graph(ls_batch* b) {
 static int enqueue_steps(ls_batch* b, double* G, const double* u, const double* v, int nsteps, hipStream_t st) {
     StageFn f[3];
     stage_fns(b->mode, b->shape, f);
+    const bool fast = b->mode == LS_MODE_FAST;
     const TileShape sh = kShapes[b->shape];
     const int nt = b->ntiles[b->shape];
-    const unsigned grid = (unsigned)std::min(nt, 256 * 16);
-    const dim3 block(sh.tx, sh.ty, 1);
+    const int ch = kChunks[b->shape];
+    const unsigned grid = fast ? (unsigned)((nt + 7) / 8 * 8) : (unsigned)std::min(nt, 256 * 16);
+    const dim3 block(fast ? lsf::kYW : sh.tx, fast ? 1 : sh.ty, 1);
     double* cur = G;
     double* a = b->buf[0];
     double* c = b->buf[1];
+    const BTile* tl = b->dtiles[b->shape];
     for (int s = 0; s < nsteps; s++) {
-        hipLaunchKernelGGL(f[0], dim3(grid), block, 0, st, b->dmem, b->dtiles[b->shape], nt, cur, cur, a, u, v);
-        hipLaunchKernelGGL(f[1], dim3(grid), block, 0, st, b->dmem, b->dtiles[b->shape], nt, a, cur, c, u, v);
-        hipLaunchKernelGGL(f[2], dim3(grid), block, 0, st, b->dmem, b->dtiles[b->shape], nt, c, cur, a, u, v);
+        hipLaunchKernelGGL(f[0], dim3(grid), block, 0, st, b->dmem, tl, nt, ch, cur, cur, a, u, v);
+        hipLaunchKernelGGL(f[1], dim3(grid), block, 0, st, b->dmem, tl, nt, ch, a, cur, c, u, v);
+        hipLaunchKernelGGL(f[2], dim3(grid), block, 0, st, b->dmem, tl, nt, ch, c, cur, a, u, v);
         double* old = cur;
         cur = a;
         a = old;
@@ -279,9 +290,11 @@ int ls_batch_create(const ls_batch_member_t* members, int32_t count, int32_t mod
         std::sort(order.begin(), order.end(), [&](int x, int y) {
             return (int64_t)members[x].nx * members[x].ny > (int64_t)members[y].nx * members[y].ny;
         });
+        const int ux = mode == LS_MODE_FAST ? lsf::kYW : kShapes[s].tx;
+        const int uy = mode == LS_MODE_FAST ? kChunks[s] : kShapes[s].ty;
         for (int m : order) {
-            const int tx = (members[m].nx + kShapes[s].tx - 1) / kShapes[s].tx;
-            const int ty = (members[m].ny + kShapes[s].ty - 1) / kShapes[s].ty;
+            const int tx = (members[m].nx + ux - 1) / ux;
+            const int ty = (members[m].ny + uy - 1) / uy;
             for (int y = 0; y < ty; y++)
                 for (int x = 0; x < tx; x++) tl.push_back(BTile{m, x, y, 0});
         }
@@ -376,8 +389,8 @@ int ls_batch_advance(ls_batch_t* b, double* G, const double* u, const double* v,
     return LS_OK;
 }
 
-// Times `probe_steps` steps with every tile shape on a scratch copy of G and
-// keeps the fastest.  G, u and v are not modified.  Synchronises the stream.
+// Times `probe_steps` steps with every tile shape (twice) on a scratch copy of
+// G and keeps the fastest.  G, u and v are not modified.  Synchronises the stream.
 int ls_batch_autotune(ls_batch_t* b, const double* G, const double* u, const double* v, int32_t probe_steps,
                       double* best_ms, void* stream) {
     if (!b || !G || !u || !v || probe_steps <= 0) return bfail(LS_ERR_INVALID, "bad arguments");
@@ -392,7 +405,10 @@ int ls_batch_autotune(ls_batch_t* b, const double* G, const double* u, const dou
     float best_t = 1e30f;
     const int keep = b->shape;
     int rc = LS_OK;
-    for (int s = 0; s < kNumShapes && rc == LS_OK; s++) {
+    // two passes over the shapes, each shape keeping its faster pass, so
+    // that the order of the probes does not pick the winner
+    for (int q = 0; q < 2 * kNumShapes && rc == LS_OK; q++) {
+        const int s = q % kNumShapes;
         b->shape = s;
         hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, st, G, scratch, b->total);
         rc = enqueue_steps(b, scratch, u, v, 1, st);  // warm-up

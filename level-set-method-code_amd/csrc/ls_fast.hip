@@ -28,6 +28,7 @@
 #include "ls_fast.hpp"
 #include "ls_fast_math.hpp"
 #include "ls_internal.hpp"
+#include "ls_ymarch.hpp"
 
 namespace lsf {
 
@@ -39,22 +40,8 @@ struct FastGeom {
     double inv6dx, inv6dy, inv6dz;
     double invdx, invdy, invdz;
     int tiles_x, tiles_y;      // block tiles per plane
-    int fuse_ghosts;           // 2D stage kernel writes the ghost images of its output
     int64_t nblocks;
 };
-
-// The a-th padded index that holds interior index q after a ghost fill
-// (a = 0 is the cell itself), and how many there are.
-__device__ __forceinline__ int n_images(int q, int n, int bc) {
-    if (bc == LS_BC_PERIODIC) return 1 + (q < 3) + (q >= n - 3);
-    if (bc == LS_BC_ZERO_GRAD) return 1 + 3 * (q == 0) + 3 * (q == n - 1);
-    return 1;
-}
-__device__ __forceinline__ int image(int q, int n, int bc, int a) {
-    if (a == 0) return q + 3;
-    if (bc == LS_BC_PERIODIC) return (a == 1 && q < 3) ? q + 3 + n : q + 3 - n;
-    return q == 0 ? a - 1 : n + 2 + a;  // zero-gradient: 0,1,2 or n+3,n+4,n+5
-}
 
 constexpr int kFX = 64;
 constexpr int kFY = 4;
@@ -69,72 +56,34 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
     return e * per + r;
 }
 
-template <int DIM>
-__device__ __forceinline__ double rhs_fast(const FastGeom& f, const double* __restrict__ P, double uc,
-                                           double vc, double wc, double s_l) {
-    const int64_t sy = f.g.sy, sz = f.g.sz;
-    double ue = uc, ve = vc, we = wc;
-    if (s_l > f.g.eps) {
-        const double gx = (P[1] - P[-1]) * f.inv2dx;
-        const double gy = (P[sy] - P[-sy]) * f.inv2dy;
-        double s2 = fma(gx, gx, gy * gy);
-        double gz = 0.0;
-        if (DIM == 3) {
-            gz = (P[sz] - P[-sz]) * f.inv2dz;
-            s2 = fma(gz, gz, s2);
-        }
-        const double k = s_l * rsqrt_nr(s2 + f.g.eps);
-        ue = fma(-k, gx, uc);
-        ve = fma(-k, gy, vc);
-        if (DIM == 3) we = fma(-k, gz, wc);
-    }
-    const double dgx = dweno_fast(P[-3], P[-2], P[-1], P[0], P[1], P[2], P[3], ue, f.inv6dx);
-    const double dgy = dweno_fast(P[-3 * sy], P[-2 * sy], P[-sy], P[0], P[sy], P[2 * sy], P[3 * sy], ve, f.inv6dy);
-    double acc = fma(ue, dgx, ve * dgy);
-    if (DIM == 3) {
-        const double dgz = dweno_fast(P[-3 * sz], P[-2 * sz], P[-sz], P[0], P[sz], P[2 * sz], P[3 * sz], we, f.inv6dz);
-        acc = fma(we, dgz, acc);
-    }
-    return -acc;
+// 2D stage: one wavefront per (64-column strip, chunk of rows), y-marching
+// (ls_ymarch.hpp).  Units are dealt XCD-contiguously, strips fastest.
+template <int STAGE>
+__global__ __launch_bounds__(kYW) void k_stage_ymarch(YGrid m, int strips, int ch, int64_t units, int64_t nblocks,
+                                                      const double* __restrict__ Gs, const double* __restrict__ Gn,
+                                                      double* __restrict__ out, const double* __restrict__ u,
+                                                      const double* __restrict__ v, const double* __restrict__ sl) {
+    __shared__ YLds L;
+    const int64_t t = xcd_tile(blockIdx.x, nblocks);
+    if (t >= units) return;
+    const int sx = (int)(t % strips), cy = (int)(t / strips);
+    const int y0 = cy * ch;
+    ymarch_strip<STAGE>(m, sx * kYW, y0, min(y0 + ch, m.ny), Gs, Gn, out, u, v, sl, L);
 }
 
-template <int DIM, int STAGE>
-__global__ __launch_bounds__(256) void k_stage_fast(FastGeom f, const double* __restrict__ Gs,
-                                                    const double* __restrict__ Gn,
-                                                    double* __restrict__ out,
-                                                    const double* __restrict__ u,
-                                                    const double* __restrict__ v,
-                                                    const double* __restrict__ w,
-                                                    const double* __restrict__ sl, double dt) {
-    const int64_t t = xcd_tile(blockIdx.x, f.nblocks);
-    if (t >= (int64_t)f.tiles_x * f.tiles_y * (DIM == 3 ? f.g.nz : 1)) return;
-    const int tx = (int)(t % f.tiles_x);
-    const int64_t r = t / f.tiles_x;
-    const int ty = (int)(r % f.tiles_y);
-    const int tz = (int)(r / f.tiles_y);
-    const int i = tx * kFX + (int)threadIdx.x;
-    const int j = ty * kFY + (int)threadIdx.y;
-    if (i >= f.g.nx || j >= f.g.ny) return;
-    const int k = DIM == 3 ? tz + f.g.ng : 0;
-    const int64_t c = (int64_t)k * f.g.sz + (int64_t)(j + f.g.ng) * f.g.sy + (i + f.g.ng);
-    const double s_l = sl ? sl[c] : f.g.s_l;
-    const double rr = rhs_fast<DIM>(f, Gs + c, u[c], v[c], DIM == 3 ? w[c] : 0.0, s_l);
-    const double gn = Gn[c];
-    double o;
-    if (STAGE == 1) o = fma(dt, rr, gn);
-    else if (STAGE == 2) o = fma(0.25 * dt, rr, fma(0.25, Gs[c], 0.75 * gn));
-    else o = fma((2.0 / 3.0) * dt, rr, fma(2.0 / 3.0, Gs[c], (1.0 / 3.0) * gn));
-    out[c] = o;
-    // ghost cells of out: written by the interior cell they copy (periodic /
-    // zero-gradient), which replaces the separate ghost-fill launch
-    if (DIM == 2 && f.fuse_ghosts && (i < 3 || i >= f.g.nx - 3 || j < 3 || j >= f.g.ny - 3)) {
-        const int nxi = n_images(i, f.g.nx, f.g.bc[0]), nyi = n_images(j, f.g.ny, f.g.bc[1]);
-        for (int b = 0; b < nyi; b++) {
-            const int64_t pj = (int64_t)image(j, f.g.ny, f.g.bc[1], b) * f.g.sy;
-            for (int e = 0; e < nxi; e++)
-                if (b | e) out[pj + image(i, f.g.nx, f.g.bc[0], e)] = o;
-        }
-    }
+// 2D stage for small grids: one thread per cell, 64x4 tiles, the y-march's
+// arithmetic (ycell).
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_stage_ycell(YGrid m, int tiles_x, int64_t tiles, int64_t nblocks,
+                                                     const double* __restrict__ Gs, const double* __restrict__ Gn,
+                                                     double* __restrict__ out, const double* __restrict__ u,
+                                                     const double* __restrict__ v, const double* __restrict__ sl) {
+    const int64_t t = xcd_tile(blockIdx.x, nblocks);
+    if (t >= tiles) return;
+    const int i = (int)(t % tiles_x) * kFX + (int)threadIdx.x;
+    const int j = (int)(t / tiles_x) * kFY + (int)threadIdx.y;
+    if (i >= m.nx || j >= m.ny) return;
+    ycell<STAGE>(m, i, j, Gs, Gn, out, u, v, sl);
 }
 
 // HCR-2 pseudo-time step (reinitialization.h:136-193) with multiplies for the
@@ -265,7 +214,7 @@ struct ZGeom {
 //    and carries it to plane k+1, which rebuilds it only where its own
 //    orientation differs.
 // A face is kept as numerator n and denominator d, F = n / (6 d), with the
-// product-form weights (see face_nd).  The three axis derivatives enter one
+// product-form weights (see face_core).  The three axis derivatives enter one
 // fraction, so a cell costs a single refined reciprocal:
 //    rhs = -(sum_a (u_a / 6h_a) N_a prod_{b != a} D_b) / prod_a D_a,
 //    N_a = n_hi d_lo - n_lo d_hi,  D_a = d_hi d_lo.
@@ -859,8 +808,6 @@ static FastGeom make_fast(const Geom& g) {
     f.invdz = 1.0 / g.dz;
     f.tiles_x = (g.nx + kFX - 1) / kFX;
     f.tiles_y = (g.ny + kFY - 1) / kFY;
-    f.fuse_ghosts = g.bc[0] != LS_BC_LIN_EXTRAP && g.bc[1] != LS_BC_LIN_EXTRAP &&
-                    (g.dim == 2 || g.bc[2] != LS_BC_LIN_EXTRAP);
     const int64_t tiles = (int64_t)f.tiles_x * f.tiles_y * (g.dim == 3 ? g.nz : 1);
     f.nblocks = ((tiles + 7) / 8) * 8;
     return f;
@@ -929,22 +876,66 @@ int launch_stage_exact3d(const Geom& g, int stage, const double* Gs, const doubl
     return launch_stage_zmarch(g, stage, Gs, Gn, out, u, v, w, sl, dt, st, true);
 }
 
+// Rows per y-march chunk.  Short chunks win on this part: 8-row chunks give
+// the most wavefronts to hide load latency behind, and the 6 halo rows each
+// chunk re-reads come mostly from L2, because the chunks of a strip run on one
+// XCD (measured over 96 uneven 2D grids: 93 us per stage with 8 rows, 104 us
+// with 32, 112 us with 16).  Returns 0 when the grid gives fewer than 2048
+// units: it then runs one thread per cell.
+int ymarch_chunk(int nx, int ny) {
+    const int64_t strips = (nx + kYW - 1) / kYW;
+    const int ch = 8;
+    return strips * ((ny + ch - 1) / ch) < 2048 ? 0 : ch;
+}
+
 template <int DIM>
 int launch_stage_fast(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
                       const double* u, const double* v, const double* w, const double* sl, double dt,
                       hipStream_t st) {
     if (DIM == 3) return launch_stage_zmarch(g, stage, Gs, Gn, out, u, v, w, sl, dt, st, false);
     const FastGeom f = make_fast(g);
-    if (f.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
-    const dim3 grid((unsigned)f.nblocks), block(kFX, kFY, 1);
-    if (stage == 1)
-        hipLaunchKernelGGL((k_stage_fast<DIM, 1>), grid, block, 0, st, f, Gs, Gn, out, u, v, w, sl, dt);
-    else if (stage == 2)
-        hipLaunchKernelGGL((k_stage_fast<DIM, 2>), grid, block, 0, st, f, Gs, Gn, out, u, v, w, sl, dt);
-    else
-        hipLaunchKernelGGL((k_stage_fast<DIM, 3>), grid, block, 0, st, f, Gs, Gn, out, u, v, w, sl, dt);
+    YGrid m;
+    m.nx = g.nx;
+    m.ny = g.ny;
+    m.bc0 = g.bc[0];
+    m.bc1 = g.bc[1];
+    m.fuse_ghosts = g.bc[0] != LS_BC_LIN_EXTRAP && g.bc[1] != LS_BC_LIN_EXTRAP;
+    m.pad = 0;
+    m.sy = g.sy;
+    m.off = 0;
+    m.s_l = g.s_l;
+    m.eps = g.eps;
+    m.dt = dt;
+    m.inv2dx = f.inv2dx;
+    m.inv2dy = f.inv2dy;
+    m.inv6dx = f.inv6dx;
+    m.inv6dy = f.inv6dy;
+    const int ch = ymarch_chunk(g.nx, g.ny);
+    if (ch == 0) {
+        const int64_t tiles = (int64_t)f.tiles_x * f.tiles_y;
+        if (f.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
+        const dim3 grid((unsigned)f.nblocks), block(kFX, kFY, 1);
+        if (stage == 1)
+            hipLaunchKernelGGL((k_stage_ycell<1>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
+        else if (stage == 2)
+            hipLaunchKernelGGL((k_stage_ycell<2>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
+        else
+            hipLaunchKernelGGL((k_stage_ycell<3>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
+    } else {
+        const int strips = (g.nx + kYW - 1) / kYW;
+        const int64_t units = (int64_t)strips * ((g.ny + ch - 1) / ch);
+        const int64_t nblocks = ((units + 7) / 8) * 8;
+        if (nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
+        const dim3 grid((unsigned)nblocks), block(kYW, 1, 1);
+        if (stage == 1)
+            hipLaunchKernelGGL((k_stage_ymarch<1>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl);
+        else if (stage == 2)
+            hipLaunchKernelGGL((k_stage_ymarch<2>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl);
+        else
+            hipLaunchKernelGGL((k_stage_ymarch<3>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl);
+    }
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast stage launch failed");
-    return f.fuse_ghosts ? 1 : 0;
+    return m.fuse_ghosts ? 1 : 0;
 }
 
 template int launch_stage_fast<2>(const Geom&, int, const double*, const double*, double*, const double*,

@@ -43,64 +43,8 @@ __device__ __forceinline__ double tsq(double a, double b, double c) {
     return s * s;
 }
 
-// Left-biased face from (v0..v4) as numerator N and denominator D with
-// face = N / (6 D); t0, t1, t2 belong to the sub-stencils centred at v1, v2, v3.
-__device__ __forceinline__ void face_nd(double v0, double v1, double v2, double v3, double v4,
-                                        double t0, double t1, double t2, double d0, double d2,
-                                        double& N, double& D) {
-    const double p0 = fma(2.0, v0, fma(-7.0, v1, 11.0 * v2));
-    const double p1 = fma(5.0, v2, fma(2.0, v3, -v1));
-    const double p2 = fma(2.0, v2, fma(5.0, v3, -v4));
-    const double w0 = d0 * (t1 * t2);
-    const double w1 = 0.6 * (t0 * t2);
-    const double w2 = d2 * (t0 * t1);
-    N = fma(w0, p0, fma(w1, p1, w2 * p2));
-    D = (w0 + w1) + w2;
-}
-
-// Upwinded WENO-5 derivative from the axis stencil m3..p3.  inv6h = 1/(6h).
-__device__ __forceinline__ double dweno_fast(double m3, double m2, double m1, double c0, double p1,
-                                             double p2, double p3, double vel, double inv6h) {
-    const bool L = vel >= 0.0;
-    const double s0 = L ? m3 : p3, s1 = L ? m2 : p2, s2 = L ? m1 : p1;
-    const double s4 = L ? p1 : m1, s5 = L ? p2 : m2;
-    const double d0 = L ? 0.1 : 0.3, d2 = L ? 0.3 : 0.1;
-    const double t1 = tsq(s0, s1, s2);
-    const double t2 = tsq(s1, s2, c0);
-    const double t3 = tsq(s2, c0, s4);
-    const double t4 = tsq(c0, s4, s5);
-    double Na, Da, Nb, Db;
-    face_nd(s0, s1, s2, c0, s4, t1, t2, t3, d0, d2, Na, Da);
-    face_nd(s1, s2, c0, s4, s5, t2, t3, t4, d0, d2, Nb, Db);
-    // left-biased: (face_b - face_a)/h, mirrored: (face_a - face_b)/h
-    const double num = fma(Nb, Da, -(Na * Db));
-    const double q = div_fast(num, Da * Db) * inv6h;
-    return L ? q : -q;
-}
-
-// Same derivative with the smoothness terms t supplied at the five centres
-// m2..p2 (tsq is symmetric in its outer arguments, so one t per point serves
-// both upwind orientations).  Bitwise equal to dweno_fast.
-__device__ __forceinline__ double dweno_t(double m3, double m2, double m1, double c0, double p1,
-                                          double p2, double p3, double tm2, double tm1, double tc0,
-                                          double tp1, double tp2, double vel, double inv6h) {
-    const bool L = vel >= 0.0;
-    const double s0 = L ? m3 : p3, s1 = L ? m2 : p2, s2 = L ? m1 : p1;
-    const double s4 = L ? p1 : m1, s5 = L ? p2 : m2;
-    const double t1 = L ? tm2 : tp2, t2 = L ? tm1 : tp1, t4 = L ? tp1 : tm1;
-    // sigma = +1 (left-biased) or -1 (mirrored): d0 = 0.2 - 0.1 sigma,
-    // d2 = 0.2 + 0.1 sigma, and the derivative carries the factor sigma.
-    const double sg = L ? 1.0 : -1.0;
-    const double d0 = fma(-0.1, sg, 0.2), d2 = fma(0.1, sg, 0.2);
-    double Na, Da, Nb, Db;
-    face_nd(s0, s1, s2, c0, s4, t1, t2, tc0, d0, d2, Na, Da);
-    face_nd(s1, s2, c0, s4, s5, t2, tc0, t4, d0, d2, Nb, Db);
-    const double num = fma(Nb, Da, -(Na * Db));
-    return div_fast(num, Da * Db) * (sg * inv6h);
-}
-
 // ---------------------------------------------------------------------------
-// Shared faces (k_stage_zface and the 2D plane kernel).  A face is kept as a
+// Shared faces (k_stage_zface and the 2D y-march).  A face is kept as a
 // numerator n and denominator d, value n / (6 d).
 struct Face {
     double n, d;

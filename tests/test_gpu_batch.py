@@ -92,6 +92,45 @@ def test_batch_autotune_keeps_results(ls, cuda):
     assert cuda.equal(G, G2)  # per-cell arithmetic does not depend on the tiling
 
 
+def test_batch_fast_shapes_match_serial(ls, cuda):
+    """Fast mode: every schedule shape (y-march chunk length) gives the same
+    bits, and so does the single-grid solver run member by member: one thread
+    per cell for the small members, its own y-march for the tall one."""
+    spec = MEMBERS + [(130, 5500, (0, 1), 0.1)]
+    members, params = [], []
+    for nx, ny, bc, s_l in spec:
+        p = ls.default_params(2, nx, nx=nx, ny=ny, bc=(bc[0], bc[1], 0), s_l=s_l, reinit_interval=0,
+                              mode=ls.MODE_FAST)
+        params.append(p)
+        members.append(dict(nx=nx, ny=ny, bc=bc, dx=p.dx, dy=p.dy, s_l=s_l, dt=0.3 * min(p.dx, p.dy)))
+    b = ls.Batch(members, mode=ls.MODE_FAST)
+    G, U, V = b.empty(), b.empty(), b.empty()
+    for q, p in enumerate(params):
+        h = (p.dy, p.dx)
+        b.view(G, q).copy_(cuda.from_numpy(bumpy_sdf(p.shape, h, 70 + q)))
+        uh, vh = rand_vel(p.shape, 70 + q, 0.8, n=2)
+        b.view(U, q).copy_(cuda.from_numpy(uh))
+        b.view(V, q).copy_(cuda.from_numpy(vh))
+    b.fill_ghosts(G)
+    steps = 6
+    results = []
+    for shape in range(4):
+        Gs = G.clone()
+        b.set_tile(shape)
+        b.advance(Gs, U, V, steps, graph=False)
+        results.append(Gs)
+    cuda.cuda.synchronize()
+    for shape in range(1, 4):
+        assert cuda.equal(results[0], results[shape]), "shape %d" % shape
+    for q, p in enumerate(params):
+        s = ls.Solver(p)
+        Gq = b.view(G, q).clone()
+        s.advance(Gq, b.view(U, q).clone(), b.view(V, q).clone(), dt=members[q]["dt"], t_final=1e30,
+                  max_steps=steps)
+        cuda.cuda.synchronize()
+        assert cuda.equal(Gq, b.view(results[0], q)), "member %d" % q
+
+
 def test_batch_rejects_bad_members(ls):
     with pytest.raises(ls.LSError):
         ls.Batch([dict(nx=4, ny=20, dx=0.1, dy=0.1, s_l=0.0, dt=1e-3)])

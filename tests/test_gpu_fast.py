@@ -1,7 +1,7 @@
 """GPU parity of the fast mode.
 
 The fast mode evaluates the same scheme with reorganised arithmetic (product-
-form WENO weights, one refined reciprocal per axis, FMA), so it is held to the
+form WENO weights, one refined reciprocal per cell, FMA), so it is held to the
 north-star tolerance rather than bit equality: 1e-12 relative L-infinity in G
 over the interior and 1e-12 (relative to the box length) in the interpolated
 zero-level-set position on every grid edge.  Reference values: the CPU oracle
@@ -34,6 +34,8 @@ CASES = [
     (3, (19, 11, 15), (0, 2, 1)),
     (3, (70, 9, 40), (1, 1, 1)),    # partial x and y tiles, z chunking, zero-gradient images
     (3, (130, 6, 5), (0, 0, 0)),    # three x tiles, ny and nz below the stencil width
+    (2, (130, 5500, 1), (0, 1, 0)),  # tall enough for the 2D y-march: partial strip, chunk edges
+    (2, (130, 5500, 1), (1, 2, 0)),  # y-march with a separate ghost fill (linear extrapolation)
 ]
 
 
