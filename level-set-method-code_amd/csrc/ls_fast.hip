@@ -15,11 +15,11 @@
 //    which equals d_k/t_k / sum_j d_j/t_j without any division;
 //  * the two face values enter the derivative as one fraction
 //        (N_b D_a - N_a D_b) / (6 h D_a D_b),
-//    so each axis costs a single reciprocal (v_rcp_f64 + Newton steps); the
-//    3D stage kernel goes further and puts all three axes over one
-//    denominator, one reciprocal per cell;
-//  * the 3D stage kernel reconstructs each face once, in the orientations
-//    its two cells need, and shares it between them (k_stage_zface);
+//    and all axes of a cell go over one denominator, so a cell costs a single
+//    reciprocal (v_rcp_f64 + Newton steps);
+//  * the stage kernels reconstruct each face once, in the orientations its
+//    two cells need, and share it between them (k_stage_zface in 3D, the
+//    y-march of ls_ymarch.hpp in 2D);
 //  * constant divisions (by 6, 2h, h) become multiplies, FMA is used freely.
 #include <hip/hip_runtime.h>
 
@@ -879,9 +879,9 @@ int launch_stage_exact3d(const Geom& g, int stage, const double* Gs, const doubl
 // Rows per y-march chunk.  Short chunks win on this part: 8-row chunks give
 // the most wavefronts to hide load latency behind, and the 6 halo rows each
 // chunk re-reads come mostly from L2, because the chunks of a strip run on one
-// XCD (measured over 96 uneven 2D grids: 93 us per stage with 8 rows, 104 us
-// with 32, 112 us with 16).  Returns 0 when the grid gives fewer than 2048
-// units: it then runs one thread per cell.
+// XCD (measured over 96 uneven 2D grids: 93 us per stage with 8 rows, 102 us
+// with 4, 107 us with 16, 110 us with 32).  Returns 0 when the grid gives
+// fewer than 2048 units: it then runs one thread per cell.
 int ymarch_chunk(int nx, int ny) {
     const int64_t strips = (nx + kYW - 1) / kYW;
     const int ch = 8;
