@@ -113,7 +113,8 @@ int ls_default_params(ls_params_t* p, int32_t dim, int32_t n);
 double ls_cfl_dt(const ls_params_t* p, double cfl, double u, double v, double w);
 
 /* Create a solver on the current HIP device; allocates two padded fields of
- * workspace plus reduction scratch. */
+ * workspace plus reduction scratch.  The solver stays bound to that device:
+ * call it with the same device current and with fields that live there. */
 int ls_create(const ls_params_t* p, ls_solver_t** out);
 int ls_destroy(ls_solver_t* s);
 /* Number of doubles in one padded field. */

@@ -435,11 +435,13 @@ int ls_create(const ls_params_t* p, ls_solver_t** out) {
     *out = nullptr;
     int rc = lsi::validate(p);
     if (rc) return rc;
+    int device = 0;
+    LS_HIP(hipGetDevice(&device));
     ls_solver* s = new ls_solver();
     s->p = *p;
     s->g = make_geom(*p);
     s->n = ls_field_size(p);
-    LS_HIP(hipGetDevice(&s->device));
+    s->device = device;
     for (int q = 0; q < 2; q++) {
         if (hipMalloc((void**)&s->buf[q], (size_t)s->n * sizeof(double)) != hipSuccess) {
             for (int r = 0; r < q; r++) (void)hipFree(s->buf[r]);
@@ -455,7 +457,12 @@ int ls_create(const ls_params_t* p, ls_solver_t** out) {
         return fail(LS_ERR_NOMEM, "allocation of reduction scratch failed");
     }
     // Zero the workspace so that never-written cells are defined.
-    for (int q = 0; q < 2; q++) LS_HIP(hipMemset(s->buf[q], 0, (size_t)s->n * sizeof(double)));
+    for (int q = 0; q < 2; q++) {
+        if (hipMemset(s->buf[q], 0, (size_t)s->n * sizeof(double)) != hipSuccess) {
+            ls_destroy(s);
+            return fail(LS_ERR_DEVICE, "hipMemset of solver workspace failed");
+        }
+    }
     *out = s;
     return LS_OK;
 }
