@@ -986,10 +986,7 @@ int launch_hcr2_fast(const Geom& g, const double* pn, const double* p0, double* 
     const FastGeom f = make_fast(g);
     if (f.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
     const dim3 grid((unsigned)f.nblocks), block(kFX, kFY, 1);
-    if (g.dim == 3)
-        hipLaunchKernelGGL((k_hcr2_fast<3>), grid, block, 0, st, f, pn, p0, pnew);
-    else
-        hipLaunchKernelGGL((k_hcr2_fast<2>), grid, block, 0, st, f, pn, p0, pnew);
+    hipLaunchKernelGGL((k_hcr2_fast<2>), grid, block, 0, st, f, pn, p0, pnew);
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast HCR-2 launch failed");
     return LS_OK;
 }

@@ -334,16 +334,19 @@ static int launch_stage(const ls_solver* s, int stage, const double* Gs, const d
                         const double* sl, double dt, hipStream_t st) {
     const Geom& g = s->g;
     if (s->p.mode == LS_MODE_FAST) return lsf::launch_stage_fast<DIM>(g, stage, Gs, Gn, out, u, v, w, sl, dt, st);
-    if constexpr (DIM == 3) return lsf::launch_stage_exact3d(g, stage, Gs, Gn, out, u, v, w, sl, dt, st);
-    dim3 grid = cell_grid(g), block(kTX, kTY, 1);
-    if (stage == 1)
-        hipLaunchKernelGGL((k_stage_exact<DIM, 1>), grid, block, 0, st, g, Gs, Gn, out, u, v, w, sl, dt);
-    else if (stage == 2)
-        hipLaunchKernelGGL((k_stage_exact<DIM, 2>), grid, block, 0, st, g, Gs, Gn, out, u, v, w, sl, dt);
-    else
-        hipLaunchKernelGGL((k_stage_exact<DIM, 3>), grid, block, 0, st, g, Gs, Gn, out, u, v, w, sl, dt);
-    LS_HIP(hipGetLastError());
-    return LS_OK;
+    if constexpr (DIM == 3) {
+        return lsf::launch_stage_exact3d(g, stage, Gs, Gn, out, u, v, w, sl, dt, st);
+    } else {
+        dim3 grid = cell_grid(g), block(kTX, kTY, 1);
+        if (stage == 1)
+            hipLaunchKernelGGL((k_stage_exact<DIM, 1>), grid, block, 0, st, g, Gs, Gn, out, u, v, w, sl, dt);
+        else if (stage == 2)
+            hipLaunchKernelGGL((k_stage_exact<DIM, 2>), grid, block, 0, st, g, Gs, Gn, out, u, v, w, sl, dt);
+        else
+            hipLaunchKernelGGL((k_stage_exact<DIM, 3>), grid, block, 0, st, g, Gs, Gn, out, u, v, w, sl, dt);
+        LS_HIP(hipGetLastError());
+        return LS_OK;
+    }
 }
 
 static int stage(const ls_solver* s, int k, const double* Gs, const double* Gn, double* out,
