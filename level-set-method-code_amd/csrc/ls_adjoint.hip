@@ -152,20 +152,15 @@ __device__ __forceinline__ int64_t off7(const AGeom& g, int q) {
     }
 }
 
-// Stage partials.  lam = cdt * lambda_out[c] (cdt = the stage's coefficient
-// of dt*RHS times dt); writes lam * dRHS_c/dG[c + off19(q)] to plane q and
-// adds the pointwise velocity and flame-speed adjoints.
-__global__ __launch_bounds__(256) void k_adj_stage(AGeom g, const double* __restrict__ Gs, const double* __restrict__ u,
-                                                   const double* __restrict__ v, const double* __restrict__ w,
-                                                   const double* __restrict__ sl, double sl0,
-                                                   const double* __restrict__ lam_out, double cdt,
-                                                   double* __restrict__ P, double* __restrict__ gu,
-                                                   double* __restrict__ gv, double* __restrict__ gw,
-                                                   double* __restrict__ gsl) {
-    const int i = blockIdx.x * kAX + threadIdx.x, j = blockIdx.y * kAY + threadIdx.y, k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
-    const int64_t c = cell(g, i, j, k);
-    const double lam = cdt * lam_out[c];
+// Stage partials of one cell.  lam = cdt * lambda_out[c] (cdt = the stage's
+// coefficient of dt*RHS times dt); out[q] = lam * dRHS_c/dG[c + off19(q)].
+// Adds the pointwise velocity and flame-speed adjoints.
+__device__ __forceinline__ void stage_partials(const AGeom& g, const double* __restrict__ Gs,
+                                               const double* __restrict__ u, const double* __restrict__ v,
+                                               const double* __restrict__ w, const double* __restrict__ sl,
+                                               double sl0, double lam, int64_t c, double (&out)[19],
+                                               double* __restrict__ gu, double* __restrict__ gv,
+                                               double* __restrict__ gw, double* __restrict__ gsl) {
     const double* G = Gs + c;
     const double s_l = sl ? sl[c] : sl0;
     const double ihx = 1.0 / g.dx, ihy = 1.0 / g.dy, ihz = 1.0 / g.dz;
@@ -186,7 +181,6 @@ __global__ __launch_bounds__(256) void k_adj_stage(AGeom g, const double* __rest
     const double dgy = dweno_grad(s7, ve, ihy, Py);
     for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sz];
     const double dgz = dweno_grad(s7, we, ihz, Pz);
-    double out[19];
     out[0] = -(ue * Px[3] + ve * Py[3] + we * Pz[3]);
     for (int q = 0; q < 6; q++) {
         const int t = q < 3 ? q : q + 1;
@@ -209,10 +203,135 @@ __global__ __launch_bounds__(256) void k_adj_stage(AGeom g, const double* __rest
         out[16] += rz * (0.5 * ihz);
         if (gsl) gsl[c] += lam * (Q * im);
     }
-    for (int q = 0; q < 19; q++) P[(int64_t)q * g.n + c] = lam * out[q];
+    for (int q = 0; q < 19; q++) out[q] *= lam;
     gu[c] -= lam * dgx;
     gv[c] -= lam * dgy;
     gw[c] -= lam * dgz;
+}
+
+// Blocks of the z-marching stage adjoint: x segments of at most 64 cells
+// (one wavefront row each), 4-row y tiles, z chunks; segments and chunks are
+// balanced, so every segment and chunk spans at least 6 cells.
+struct ZMGeom {
+    int nsx, nty, nch;
+};
+
+__device__ __forceinline__ int split_at(int n, int parts, int q) { return (int)(((int64_t)n * q) / parts); }
+
+// Stage partials, summed in the block along x and z.  A block walks its z
+// chunk one plane at a time; each cell computes its 19 partials (stage_partials).
+//   * x: the 6 x partials of a row go through LDS; every cell of the segment
+//     sums the ones that land on it, and lanes 0..5 sum the ones that leave
+//     the segment (3 cells each side) into the spill plane EX.
+//   * z: each cell carries accumulators for the 7 target planes k-3..k+3 of
+//     its column; a target plane is complete three planes later and goes to
+//     `core`, or, below and above the chunk, to the spill plane EZ.
+//   * y: the 6 y partials go to planes Py[0..5] as P_q[c] (gathered later).
+// A spill position has one writer per pass, because segments and chunks span
+// at least 6 cells, and every pass writes the same positions; positions never
+// written stay zero.  The sums run in a fixed order: the result is
+// deterministic.
+__global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_adj_zm(AGeom g, ZMGeom zm, const double* __restrict__ Gs,
+                                                     const double* __restrict__ u, const double* __restrict__ v,
+                                                     const double* __restrict__ w, const double* __restrict__ sl,
+                                                     double sl0, const double* __restrict__ lam_out, double cdt,
+                                                     double* __restrict__ core, double* __restrict__ Py,
+                                                     double* __restrict__ EX, double* __restrict__ EZ,
+                                                     double* __restrict__ gu, double* __restrict__ gv,
+                                                     double* __restrict__ gw, double* __restrict__ gsl) {
+    __shared__ double sX[kAY][6][kAX];
+    const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+    int b = (int)blockIdx.x;
+    const int sxi = b % zm.nsx;
+    b /= zm.nsx;
+    const int ty = b % zm.nty, ch = b / zm.nty;
+    const int xs = split_at(g.nx, zm.nsx, sxi), L = split_at(g.nx, zm.nsx, sxi + 1) - xs;
+    const int kb = split_at(g.nz, zm.nch, ch), ke = split_at(g.nz, zm.nch, ch + 1);
+    const int i = xs + lx, j = ty * kAY + ly;
+    const bool row = j < g.ny;
+    const bool act = row && lx < L;
+    double zacc[7];  // targets k-3 .. k+3 of the current source plane k
+#pragma unroll
+    for (int q = 0; q < 7; q++) zacc[q] = 0.0;
+    for (int k = kb; k < ke; k++) {
+        if (act) {
+            const int64_t c = cell(g, i, j, k);
+            double out[19];
+            stage_partials(g, Gs, u, v, w, sl, sl0, cdt * lam_out[c], c, out, gu, gv, gw, gsl);
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                sX[ly][q][lx] = out[1 + q];
+                Py[(int64_t)q * g.n + c] = out[7 + q];
+            }
+            // z partials: out[13 + q] lands on plane k + o, o = -3,-2,-1,1,2,3
+            zacc[3] += out[0];
+            zacc[0] += out[13];
+            zacc[1] += out[14];
+            zacc[2] += out[15];
+            zacc[4] += out[16];
+            zacc[5] += out[17];
+            zacc[6] += out[18];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; q++) sX[ly][q][lx] = 0.0;
+        }
+        __syncthreads();
+        // x partial q of source s lands on s + o, o = -3,-2,-1,1,2,3
+        if (act) {
+            double xsum = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const int s = lx - (q < 3 ? q - 3 : q - 2);
+                if (s >= 0 && s < L) xsum += sX[ly][q][s];
+            }
+            zacc[3] += xsum;
+        }
+        if (row && lx < 6) {  // the 3 targets on each side of the segment
+            const int p = lx < 3 ? lx - 3 : L + lx - 3;
+            double e = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const int s = p - (q < 3 ? q - 3 : q - 2);
+                if (s >= 0 && s < L) e += sX[ly][q][s];
+            }
+            EX[cell(g, xs + p, j, k)] = e;
+        }
+        __syncthreads();
+        if (act) {  // target plane k-3 has all of this chunk's partials
+            const int kt = k - 3;
+            (kt >= kb ? core : EZ)[cell(g, i, j, kt)] = zacc[0];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) zacc[q] = zacc[q + 1];
+        zacc[6] = 0.0;
+    }
+    if (act) {  // targets ke-3 .. ke+2
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int kt = ke - 3 + q;
+            (q < 3 ? core : EZ)[cell(g, i, j, kt)] = zacc[q];
+        }
+    }
+}
+
+// target[p] = a * self[p] + core[p] + EX[p] + EZ[p] + sum_q Py_q[p - off_q]
+// (+ target[p] if accumulate) over every padded cell p: the stage adjoint's
+// y gather plus the block sums of k_adj_zm.  Planes are zero where nothing
+// was written, so a source index that leaves the array contributes nothing.
+__global__ __launch_bounds__(256) void k_adj_combine(AGeom g, const double* __restrict__ core,
+                                                     const double* __restrict__ Py, const double* __restrict__ EX,
+                                                     const double* __restrict__ EZ, const double* __restrict__ self,
+                                                     double a, double* __restrict__ target, int accumulate) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n; p += (int64_t)gridDim.x * blockDim.x) {
+        double acc = self ? a * self[p] : 0.0;
+        acc += core[p] + EX[p] + EZ[p];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int64_t src = p - (int64_t)(q < 3 ? q - 3 : q - 2) * g.sy;
+            if (src >= 0 && src < g.n) acc += Py[(int64_t)q * g.n + src];
+        }
+        target[p] = accumulate ? target[p] + acc : acc;
+    }
 }
 
 // target[p] = a * self[p] + sum_q P_q[p - off_q] (+ target[p] if accumulate)
@@ -437,7 +556,8 @@ struct ls_adjoint {
     AGeom g;
     int nsteps = 0, ckpt = 0, nseg = 0, iters = 0;
     size_t fb = 0;
-    double* P = nullptr;        // 19 partial planes (ghost regions stay zero)
+    ZMGeom zm;                  // block layout of the z-marching stage adjoint
+    double* P = nullptr;        // 19 partial planes (ghost regions stay zero; 14..16 are the stage adjoint's)
     double* lam = nullptr;      // adjoint of the current state
     double* lam2 = nullptr;     // scratch adjoints
     double* lam3 = nullptr;
@@ -486,6 +606,23 @@ int ls_adjoint_create(ls_solver_t* s, int32_t nsteps, int32_t ckpt, ls_adjoint_t
     a->nseg = (nsteps + a->ckpt - 1) / a->ckpt;
     a->iters = (p.reinit_interval > 0) ? p.reinit_iters : 0;
     a->fb = (size_t)a->g.n * sizeof(double);
+    // Balanced x segments of at most 64 cells and z chunks of at least 6
+    // planes; as many chunks as fill the GPU's resident blocks once (one round,
+    // no tail), as the occupancy query reports for this kernel.
+    a->zm.nsx = (p.nx + kAX - 1) / kAX;
+    a->zm.nty = (p.ny + kAY - 1) / kAY;
+    {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adj_zm, kAX * kAY, 0) != hipSuccess) {
+            delete a;
+            return lsi::set_error(LS_ERR_DEVICE, "device query for the adjoint failed");
+        }
+        const int64_t cols = (int64_t)a->zm.nsx * a->zm.nty;
+        const int64_t fill = std::max<int64_t>(1, ((int64_t)cus * std::max(per_cu, 1) + cols / 2) / cols);
+        a->zm.nch = (int)std::max<int64_t>(1, std::min<int64_t>(fill, p.nz / 6));
+    }
     auto field = [&](size_t count) -> double* {
         double* d = nullptr;
         if (hipMalloc((void**)&d, count * a->fb) != hipSuccess) return nullptr;
@@ -569,6 +706,13 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
     const double sl0 = a->p.s_l;
     const dim3 cgrid((g.nx + kAX - 1) / kAX, (g.ny + kAY - 1) / kAY, g.nz), cblk(kAX, kAY, 1);
     const dim3 lgrid(1024), lblk(256);
+    // z-marching stage adjoint: its y planes are P[0..5], its block sums and
+    // spills P[14..16] (planes nothing else writes)
+    const ZMGeom zm = a->zm;
+    const dim3 zgrid((unsigned)(zm.nsx * zm.nty * zm.nch));
+    double* core = a->P + 14 * g.n;
+    double* EX = a->P + 15 * g.n;
+    double* EZ = a->P + 16 * g.n;
     int rc = LS_OK;
 #define ADJ_HIP(call)                                                          \
     do {                                                                       \
@@ -632,12 +776,12 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
         const double B = stage == 1 ? 0.0 : (stage == 2 ? 0.25 : 2.0 / 3.0);
         const double C = stage == 1 ? 1.0 : (stage == 2 ? 0.25 : 2.0 / 3.0);
         hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, lam_in);
-        hipLaunchKernelGGL(k_adj_stage, cgrid, cblk, 0, st, g, Gs, u, v, w, sl_field, sl0, lam_in, C * dt, a->P, gu,
-                           gv, gw, gsl);
+        hipLaunchKernelGGL(k_adj_zm, zgrid, cblk, 0, st, g, zm, Gs, u, v, w, sl_field, sl0, lam_in, C * dt, core,
+                           a->P, EX, EZ, gu, gv, gw, gsl);
         if (stage == 1) {
-            hipLaunchKernelGGL((k_gather<19>), lgrid, lblk, 0, st, g, a->P, lam_in, A + B, a->acc, 1);
+            hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, a->P, EX, EZ, lam_in, A + B, a->acc, 1);
         } else {
-            hipLaunchKernelGGL((k_gather<19>), lgrid, lblk, 0, st, g, a->P, lam_in, B, lam_s, 0);
+            hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, a->P, EX, EZ, lam_in, B, lam_s, 0);
             hipLaunchKernelGGL(k_axpy, lgrid, lblk, 0, st, g.n, A, lam_in, a->acc);
         }
         ADJ_HIP(hipGetLastError());
