@@ -274,8 +274,9 @@ int ls_batch_autotune(ls_batch_t* b, const double* G, const double* u, const dou
  *
  * ls_adjoint_create binds a context to a 3D solver (periodic or zero-gradient
  * boundaries) and allocates its workspace: 26 padded fields, plus one per
- * checkpoint (every ckpt steps), ckpt + 1 segment states, ckpt
- * pre-reinitialisation states and reinit_iters - 1 HCR-2 iterates. */
+ * checkpoint (every ckpt steps), ckpt + 1 segment states, 2 ckpt stage
+ * fields, ckpt pre-reinitialisation states and reinit_iters - 1 HCR-2
+ * iterates. */
 typedef struct ls_adjoint ls_adjoint_t;
 int ls_adjoint_create(ls_solver_t* s, int32_t nsteps, int32_t ckpt, ls_adjoint_t** out);
 int ls_adjoint_destroy(ls_adjoint_t* a);

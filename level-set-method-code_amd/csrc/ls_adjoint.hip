@@ -570,6 +570,8 @@ struct ls_adjoint {
     std::vector<double*> seg;   // states of one segment (ckpt + 1)
     std::vector<double*> pre;   // pre-reinitialisation states of one segment
     std::vector<double*> hit;   // HCR-2 iterates 1..iters-1 (hit[0] unused)
+    std::vector<double*> g1s;   // stage fields G1, G2 of each step of one segment
+    std::vector<double*> g2s;
     std::vector<double*> all;   // every device allocation
 };
 
@@ -640,6 +642,8 @@ int ls_adjoint_create(ls_solver_t* s, int32_t nsteps, int32_t ckpt, ls_adjoint_t
     for (int q = 0; ok && a->iters > 0 && q < a->ckpt; q++) ok = (a->pre.emplace_back(field(1)) != nullptr);
     if (a->iters > 0) a->hit.push_back(nullptr);
     for (int q = 1; ok && q < a->iters; q++) ok = (a->hit.emplace_back(field(1)) != nullptr);
+    for (int q = 0; ok && q < a->ckpt; q++)
+        ok = (a->g1s.emplace_back(field(1)) != nullptr) && (a->g2s.emplace_back(field(1)) != nullptr);
     if (ok) ok = hipHostMalloc((void**)&a->loss_host, sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
         release(a);
@@ -725,13 +729,14 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
 
     auto reinit_after = [&](int n) { return iters > 0 && n > 0 && n % ri == 0; };
 
-    // Forward step n from `from` into `to`; with reinitialisation the RK3
-    // result goes to `prestate` and the HCR-2 iterates through hit[].
-    auto fwd_step = [&](const double* from, double* to, double* prestate, int n) -> int {
-        ADJ_RC(ls_rk3_stage(s, 1, from, from, a->G1, u, v, w, sl_field, dt, st));
-        ADJ_RC(ls_rk3_stage(s, 2, a->G1, from, a->G2, u, v, w, sl_field, dt, st));
-        if (!reinit_after(n)) return ls_rk3_stage(s, 3, a->G2, from, to, u, v, w, sl_field, dt, st);
-        ADJ_RC(ls_rk3_stage(s, 3, a->G2, from, prestate, u, v, w, sl_field, dt, st));
+    // Forward step n from `from` into `to`, its stage fields into g1, g2; with
+    // reinitialisation the RK3 result goes to `prestate` and the HCR-2
+    // iterates through hit[].
+    auto fwd_step = [&](const double* from, double* to, double* prestate, int n, double* g1, double* g2) -> int {
+        ADJ_RC(ls_rk3_stage(s, 1, from, from, g1, u, v, w, sl_field, dt, st));
+        ADJ_RC(ls_rk3_stage(s, 2, g1, from, g2, u, v, w, sl_field, dt, st));
+        if (!reinit_after(n)) return ls_rk3_stage(s, 3, g2, from, to, u, v, w, sl_field, dt, st);
+        ADJ_RC(ls_rk3_stage(s, 3, g2, from, prestate, u, v, w, sl_field, dt, st));
         const double* in = prestate;
         for (int it = 0; it < iters; it++) {
             double* o = it + 1 == iters ? to : a->hit[it + 1];
@@ -748,7 +753,7 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
     for (int n = 0; n < a->nsteps; n++) {
         const bool to_ck = (n + 1) % ckpt == 0 && n + 1 < a->nsteps;
         double* nxt = to_ck ? a->ck[(n + 1) / ckpt] : (cur == a->seg[0] ? a->seg[1] : a->seg[0]);
-        ADJ_RC(fwd_step(cur, nxt, iters > 0 ? a->pre[0] : nullptr, n));
+        ADJ_RC(fwd_step(cur, nxt, iters > 0 ? a->pre[0] : nullptr, n, a->G1, a->G2));
         cur = nxt;
     }
     double* lamT = grad ? a->lam : nullptr;
@@ -813,7 +818,8 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
         const double* start = a->ck[sg];
         for (int n = n0; n < n1; n++) {
             const double* from = n == n0 ? start : a->seg[n - n0];
-            ADJ_RC(fwd_step(from, a->seg[n - n0 + 1], iters > 0 ? a->pre[n - n0] : nullptr, n));
+            ADJ_RC(fwd_step(from, a->seg[n - n0 + 1], iters > 0 ? a->pre[n - n0] : nullptr, n, a->g1s[n - n0],
+                            a->g2s[n - n0]));
         }
         for (int n = n1 - 1; n >= n0; n--) {
             const int q = n - n0;
@@ -826,11 +832,9 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
                 }
                 ADJ_RC(hcr2_vjp(a->pre[q]));
             }
-            ADJ_RC(ls_rk3_stage(s, 1, Gn, Gn, a->G1, u, v, w, sl_field, dt, st));
-            ADJ_RC(ls_rk3_stage(s, 2, a->G1, Gn, a->G2, u, v, w, sl_field, dt, st));
             ADJ_HIP(hipMemsetAsync(a->acc, 0, fb, st));
-            ADJ_RC(stage_vjp(3, a->G2, a->lam, a->lam2));   // lam2: adjoint of G2
-            ADJ_RC(stage_vjp(2, a->G1, a->lam2, a->lam3));  // lam3: adjoint of G1
+            ADJ_RC(stage_vjp(3, a->g2s[q], a->lam, a->lam2));   // lam2: adjoint of G2
+            ADJ_RC(stage_vjp(2, a->g1s[q], a->lam2, a->lam3));  // lam3: adjoint of G1
             ADJ_RC(stage_vjp(1, Gn, a->lam3, nullptr));
             std::swap(a->lam, a->acc);  // adjoint of G_n
         }
