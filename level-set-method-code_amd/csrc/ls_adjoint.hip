@@ -22,11 +22,12 @@
 // are deterministic, so every branch above is re-derived from a state that is
 // bitwise the forward state.
 //
-// Each per-cell kernel writes the partial derivatives of its own output into
-// per-offset planes (P_q[c] = lambda_c * d out_c / d G[c + off_q]); a gather
-// kernel sums, at every target cell, the planes of the cells whose stencil
-// holds it.  Fixed summation order, no atomics: the gradient is
-// deterministic.
+// Each cell's partial derivatives P_q[c] = lambda_c * d out_c / d G[c + off_q]
+// must be summed at the target cells c + off_q.  The stage adjoint (k_adj_zm)
+// sums its x and z partials inside the block and leaves the y partials and
+// the block-edge spills to a combine kernel; the HCR-2 adjoint writes its
+// 7-point partials to planes that a gather kernel sums.  Fixed summation
+// order, no atomics: the gradient is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -132,14 +133,7 @@ __device__ __forceinline__ double dweno_grad(const double* s7, double vel, doubl
     return (fb - fa) * ih;
 }
 
-// Plane offsets.  19-point stencil: 0 centre, 1..6 x -3,-2,-1,+1,+2,+3,
-// 7..12 y, 13..18 z.  7-point stencil: 0 centre, x-, x+, y-, y+, z-, z+.
-__device__ __forceinline__ int64_t off19(const AGeom& g, int q) {
-    if (q == 0) return 0;
-    const int a = (q - 1) / 6, r = (q - 1) % 6;
-    const int o = r < 3 ? r - 3 : r - 2;
-    return a == 0 ? (int64_t)o : (a == 1 ? o * g.sy : o * g.sz);
-}
+// Plane offsets of the 7-point stencil: 0 centre, x-, x+, y-, y+, z-, z+.
 __device__ __forceinline__ int64_t off7(const AGeom& g, int q) {
     switch (q) {
         case 0: return 0;
@@ -153,7 +147,8 @@ __device__ __forceinline__ int64_t off7(const AGeom& g, int q) {
 }
 
 // Stage partials of one cell.  lam = cdt * lambda_out[c] (cdt = the stage's
-// coefficient of dt*RHS times dt); out[q] = lam * dRHS_c/dG[c + off19(q)].
+// coefficient of dt*RHS times dt); out[q] = lam * dRHS_c/dG[c + o_q] with
+// q = 0 the centre, 1..6 the x offsets -3,-2,-1,+1,+2,+3, 7..12 y, 13..18 z.
 // Adds the pointwise velocity and flame-speed adjoints.
 __device__ __forceinline__ void stage_partials(const AGeom& g, const double* __restrict__ Gs,
                                                const double* __restrict__ u, const double* __restrict__ v,
@@ -334,16 +329,15 @@ __global__ __launch_bounds__(256) void k_adj_combine(AGeom g, const double* __re
     }
 }
 
-// target[p] = a * self[p] + sum_q P_q[p - off_q] (+ target[p] if accumulate)
+// target[p] = a * self[p] + sum_q P_q[p - off7(q)] (+ target[p] if accumulate)
 // over every padded cell p.  The planes are zero outside the interior, so a
 // source index that leaves the array or wraps a row contributes nothing.
-template <int NQ>
-__global__ __launch_bounds__(256) void k_gather(AGeom g, const double* __restrict__ P, const double* __restrict__ self,
-                                                double a, double* __restrict__ target, int accumulate) {
+__global__ __launch_bounds__(256) void k_gather7(AGeom g, const double* __restrict__ P, const double* __restrict__ self,
+                                                 double a, double* __restrict__ target, int accumulate) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n; p += (int64_t)gridDim.x * blockDim.x) {
         double acc = self ? a * self[p] : 0.0;
-        for (int q = 0; q < NQ; q++) {
-            const int64_t src = p - (NQ == 19 ? off19(g, q) : off7(g, q));
+        for (int q = 0; q < 7; q++) {
+            const int64_t src = p - off7(g, q);
             if (src >= 0 && src < g.n) acc += P[(int64_t)q * g.n + src];
         }
         target[p] = accumulate ? target[p] + acc : acc;
@@ -801,9 +795,9 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
             const double* in = it == 0 ? phi0 : a->hit[it];
             hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, a->lam);
             hipLaunchKernelGGL(k_adj_hcr2, cgrid, cblk, 0, st, g, in, phi0, a->lam, a->P, a->P + 7 * g.n);
-            hipLaunchKernelGGL((k_gather<7>), lgrid, lblk, 0, st, g, a->P + 7 * g.n, (const double*)nullptr, 0.0,
+            hipLaunchKernelGGL(k_gather7, lgrid, lblk, 0, st, g, a->P + 7 * g.n, (const double*)nullptr, 0.0,
                                a->lam2, 1);
-            hipLaunchKernelGGL((k_gather<7>), lgrid, lblk, 0, st, g, a->P, (const double*)nullptr, 0.0, a->lam3, 0);
+            hipLaunchKernelGGL(k_gather7, lgrid, lblk, 0, st, g, a->P, (const double*)nullptr, 0.0, a->lam3, 0);
             std::swap(a->lam, a->lam3);
         }
         // the first iterate is phi_0 itself
