@@ -192,6 +192,13 @@ int ls_check_finite(ls_solver_t* s, const double* G, void* stream);
  * doubles with the y-ghost rows replaced by the adjacent interior row. */
 int ls_write_G(const ls_params_t* p, const double* host_G, const char* path);
 
+/* Write a HOST copy of a padded field as legacy ASCII VTK structured points of
+ * the interior (x fastest, then y, then z), as saveFieldVTK
+ * (Local_code/level-set_MPI_2D/include/io.h:119-180) and saveFieldVTK3D
+ * (Local_code/level-set_MPI_3D/include/io.h:145-215) write it: ORIGIN is
+ * (x0, y0[, z0]), SPACING (dx, dy[, dz]), one value per line in %e. */
+int ls_write_G_vtk(const ls_params_t* p, const double* host_G, const char* path);
+
 /* Synthetic inputs generated on the device (padded arrays, ghosts included).
  *   LS_CASE_CIRCLE2D        circle r=0.25 at (0.5,0.5), u=(0.5,0)
  *   LS_CASE_SPHERE_SWIRL3D  sphere r=0.2 at (0.5,0.5,0.5),

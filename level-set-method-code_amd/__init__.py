@@ -104,6 +104,7 @@ def lib():
     L.ls_diag.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
     L.ls_check_finite.argtypes = [vp, vp, vp]
     L.ls_write_G.argtypes = [P, vp, C.c_char_p]
+    L.ls_write_G_vtk.argtypes = [P, vp, C.c_char_p]
     L.ls_init_case.argtypes = [P, C.c_int32, C.c_uint64, C.c_double, vp, vp, vp, vp, vp]
     L.ls_last_error.restype = C.c_char_p
     BM = C.POINTER(BatchMember)
@@ -297,6 +298,15 @@ def write_G(p, host_array, path):
     import numpy as np
     a = np.ascontiguousarray(host_array, dtype=np.float64)
     check(lib().ls_write_G(C.byref(p), C.c_void_p(a.ctypes.data), str(path).encode()))
+
+
+def write_G_vtk(p, host_array, path):
+    """Legacy ASCII VTK of the interior (the reference's saveFieldVTK[3D])."""
+    import numpy as np
+    a = np.ascontiguousarray(host_array, dtype=np.float64)
+    if tuple(a.shape) != tuple(p.shape):
+        raise ValueError("field shape %s != %s" % (tuple(a.shape), tuple(p.shape)))
+    check(lib().ls_write_G_vtk(C.byref(p), C.c_void_p(a.ctypes.data), str(path).encode()))
 
 
 class Batch:

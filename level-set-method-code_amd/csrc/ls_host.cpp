@@ -85,6 +85,41 @@ int ls_write_G(const ls_params_t* p, const double* host_G, const char* path) {
     return LS_OK;
 }
 
+// saveFieldVTK / saveFieldVTK3D (2D io.h:119-180, 3D io.h:145-215): the
+// header lines and number formats of the reference, interior values only.
+int ls_write_G_vtk(const ls_params_t* p, const double* host_G, const char* path) {
+    int rc = lsi::validate(p);
+    if (rc) return rc;
+    if (!host_G || !path) return lsi::set_error(LS_ERR_INVALID, "NULL argument");
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return lsi::set_error(LS_ERR_IO, "cannot open output file");
+    const int ng = p->nghost;
+    const bool d3 = p->dim == 3;
+    const long long nxt = p->nx + 2 * ng, nyt = p->ny + 2 * ng;
+    const long long nz = d3 ? p->nz : 1;
+    bool ok = std::fprintf(fp, "# vtk DataFile Version 3.0\n%s\nASCII\nDATASET STRUCTURED_POINTS\n",
+                           d3 ? "G-equation Level-Set Field 3D" : "G-equation Level-Set Field") > 0;
+    if (d3) {
+        ok = ok && std::fprintf(fp, "DIMENSIONS %d %d %d\n", p->nx, p->ny, p->nz) > 0 &&
+             std::fprintf(fp, "ORIGIN %f %f %f\n", p->x0, p->y0, p->z0) > 0 &&
+             std::fprintf(fp, "SPACING %f %f %f\n", p->dx, p->dy, p->dz) > 0;
+    } else {
+        ok = ok && std::fprintf(fp, "DIMENSIONS %d %d 1\n", p->nx, p->ny) > 0 &&
+             std::fprintf(fp, "ORIGIN %f %f 0\n", p->x0, p->y0) > 0 &&
+             std::fprintf(fp, "SPACING %f %f 1\n", p->dx, p->dy) > 0;
+    }
+    ok = ok && std::fprintf(fp, "POINT_DATA %lld\nSCALARS G double 1\nLOOKUP_TABLE default\n",
+                            (long long)p->nx * p->ny * nz) > 0;
+    for (long long k = 0; ok && k < nz; k++)
+        for (long long j = 0; ok && j < p->ny; j++) {
+            const double* row = host_G + ((d3 ? k + ng : 0) * nyt + j + ng) * nxt + ng;
+            for (long long i = 0; ok && i < p->nx; i++) ok = std::fprintf(fp, "%e\n", row[i]) > 0;
+        }
+    if (std::fclose(fp) != 0) ok = false;
+    if (!ok) return lsi::set_error(LS_ERR_IO, "short write");
+    return LS_OK;
+}
+
 // save_G_to_bin (sample/main.cpp:34-51): the interior rows only, no header.
 int ls_sample_write_G(int32_t nx, int32_t ny, const double* host_G, const char* path) {
     if (nx < 6 || ny < 6) return lsi::set_error(LS_ERR_INVALID, "the sample scheme needs nx, ny >= 6");

@@ -157,7 +157,11 @@ int main(int argc, char** argv) {
                 dim == 3 ? "Volume" : "Area", a0, dim == 3 ? "Volume" : "Area", a1);
     std::printf("Timing Results:\n  Total steps: %d\n  Wall time: %.3f s\n  Time per step: %.3f ms\n", step, el,
                 el / step * 1e3);
-    if (save) saveFieldBinary((outdir + "/G_final.bin").c_str(), G, sp);
+    if (save) {
+        saveFieldBinary((outdir + "/G_final.bin").c_str(), G, sp);
+        // the 3D driver also writes a VTK copy (Local_code/level-set_MPI_3D/src/main.cpp:377-380)
+        if (dim == 3) saveFieldVTK3D((outdir + "/G_final.vtk").c_str(), G, sp);
+    }
     for (auto ptr : f) hipFree(ptr);
     hipFree(flag);
     return 0;

@@ -13,8 +13,8 @@
 //   reinitializeWithSwap[3D]             reinitialization.h:321-360 / 3D :199-228
 //   computeInterfaceCrossings[3D]        reinitialization.h:75-130 / 3D :46-89
 //   calculateTimeStep, initDefaultParams config.h:166-263 / 3D config.h:180-293
-//   computeL2Error[3D], computeInterfaceArea, computeInterfaceVolume, saveFieldBinary[3D]
-//                                        io.h:34-246 / 3D io.h:34-288
+//   computeL2Error[3D], computeInterfaceArea, computeInterfaceVolume, saveFieldBinary[3D],
+//   saveFieldVTK[3D]                     io.h:34-246 / 3D io.h:34-288
 //
 // Differences a caller can observe: G_rhs is not written (the RHS lives only
 // in registers inside the fused stage kernel), and the single-GPU solver has
@@ -242,4 +242,17 @@ inline bool saveFieldBinary(const char* filename, const double* G, SimParams& sp
 }
 inline bool saveFieldBinary3D(const char* filename, const double* G, SimParams& sp, double = 0.0) {
     return saveFieldBinary(filename, G, sp);
+}
+
+// saveFieldVTK[3D]: device field -> host -> legacy ASCII VTK of the interior.
+inline bool saveFieldVTK(const char* filename, const double* G, SimParams& sp) {
+    ls_params_t p = lsref::to_params(sp);
+    std::vector<double> h((size_t)ls_field_size(&p));
+    if (hipMemcpyAsync(h.data(), G, h.size() * sizeof(double), hipMemcpyDeviceToHost, sp.stream) != hipSuccess ||
+        hipStreamSynchronize(sp.stream) != hipSuccess)
+        return false;
+    return ls_write_G_vtk(&p, h.data(), filename) == LS_OK;
+}
+inline bool saveFieldVTK3D(const char* filename, const double* G, SimParams& sp) {
+    return saveFieldVTK(filename, G, sp);
 }

@@ -514,6 +514,42 @@ int lso_write_G(const lso_grid* g, const double* G, const char* path) {
     return 0;
 }
 
+/* saveFieldVTK (Local_code/level-set_MPI_2D/include/io.h:119-180) and
+ * saveFieldVTK3D (Local_code/level-set_MPI_3D/include/io.h:145-215): legacy
+ * ASCII VTK structured points of the interior, x fastest. */
+int lso_write_vtk(const lso_grid* g, const double* G, double x0, double y0, double z0, const char* path) {
+    const int64_t nxt = nxt_(g), nyt = nyt_(g);
+    const int ng = g->ng;
+    FILE* fp = fopen(path, "w");
+    if (!fp) return -1;
+    fprintf(fp, "# vtk DataFile Version 3.0\n");
+    fprintf(fp, g->dim == 3 ? "G-equation Level-Set Field 3D\n" : "G-equation Level-Set Field\n");
+    fprintf(fp, "ASCII\n");
+    fprintf(fp, "DATASET STRUCTURED_POINTS\n");
+    if (g->dim == 3) {
+        fprintf(fp, "DIMENSIONS %d %d %d\n", g->nx, g->ny, g->nz);
+        fprintf(fp, "ORIGIN %f %f %f\n", x0, y0, z0);
+        fprintf(fp, "SPACING %f %f %f\n", g->dx, g->dy, g->dz);
+        fprintf(fp, "POINT_DATA %d\n", g->nx * g->ny * g->nz);
+    } else {
+        fprintf(fp, "DIMENSIONS %d %d 1\n", g->nx, g->ny);
+        fprintf(fp, "ORIGIN %f %f 0\n", x0, y0);
+        fprintf(fp, "SPACING %f %f 1\n", g->dx, g->dy);
+        fprintf(fp, "POINT_DATA %d\n", g->nx * g->ny);
+    }
+    fprintf(fp, "SCALARS G double 1\n");
+    fprintf(fp, "LOOKUP_TABLE default\n");
+    const int nz = g->dim == 3 ? g->nz : 1;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < g->ny; j++)
+            for (int i = 0; i < g->nx; i++) {
+                const int64_t kk = g->dim == 3 ? k + ng : 0;
+                fprintf(fp, "%e\n", G[(kk * nyt + j + ng) * nxt + i + ng]);
+            }
+    fclose(fp);
+    return 0;
+}
+
 /* ---- reference initial conditions (log known-answer cases) ----------------
  * Coordinates x = (i - ng) * dx on the unit box (config.h:278-294).
  * Zalesak disk: Local_code/level-set_MPI_2D/include/initial_conditions.h:91-117

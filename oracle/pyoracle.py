@@ -185,6 +185,10 @@ class Oracle(_Lib):
     def write_G(self, g, G, path):
         return self.lib.lso_write_G(C.byref(g), _p(G), str(path).encode())
 
+    def write_vtk(self, g, G, path, origin=(0.0, 0.0, 0.0)):
+        o = [C.c_double(x) for x in origin]
+        return self.lib.lso_write_vtk(C.byref(g), _p(G), o[0], o[1], o[2], str(path).encode())
+
 
 def ref_available(dim=2, fma=False):
     return os.path.exists(os.path.join(HERE, "_ref", "libref%dd%s.so" % (dim, "_fma" if fma else "")))
@@ -266,6 +270,9 @@ class Reference(_Lib):
 
     def write_G(self, g, G, path):
         return self.lib.ref_write_G(C.byref(g), _p(G), str(path).encode())
+
+    def write_vtk(self, g, G, path):
+        return self.lib.ref_write_vtk(C.byref(g), _p(G), str(path).encode())
 
 
 class Tangent:

@@ -282,6 +282,17 @@ int ref_write_G(const ref_grid* g, const double* G, const char* path) {
 #endif
 }
 
+// The reference's VTK writer (x_min = y_min = z_min = 0 here).
+int ref_write_vtk(const ref_grid* g, const double* G, const char* path) {
+    ensure_mpi();
+    SimParams p = mk(g);
+#if REF_DIM == 2
+    return saveFieldVTK(path, G, p) ? 0 : -1;
+#else
+    return saveFieldVTK3D(path, G, p) ? 0 : -1;
+#endif
+}
+
 // Reference initial conditions on the padded grid (x0 = y0 = z0 = 0).
 // 2D: case 1 circle (0.25,0.5) r 0.15, case 2 Zalesak disk + rotation
 // (omega = 2 pi / t_final).  3D: case 0 sphere-deformation test (G only;

@@ -95,6 +95,45 @@ def test_write_G_matches_oracle_bytes(ls, oracle, tmp_path, dim):
     assert a.read_bytes() == b.read_bytes()
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_write_G_vtk_matches_reference(ls, oracle, tmp_path, dim):
+    """saveFieldVTK / saveFieldVTK3D: byte-equal to the restatement and, where
+    oracle/_ref is built, to the reference's own writer (origin 0 there)."""
+    from oracle.pyoracle import Reference, ref_available
+    n = (9, 7, 5)
+    p = ls.default_params(dim, 9, nx=n[0], ny=n[1], nz=n[2] if dim == 3 else 1)
+    p.x0 = p.y0 = p.z0 = 0.0
+    g = make_grid(dim, *n, bc=tuple(p.bc))
+    assert (g.dx, g.dy) == (p.dx, p.dy)
+    G = np.random.default_rng(10 + dim).standard_normal(g.shape)
+    a, b = tmp_path / "a.vtk", tmp_path / "b.vtk"
+    ls.write_G_vtk(p, G, a)
+    oracle.write_vtk(g, G, b)
+    assert a.read_bytes() == b.read_bytes()
+    if ref_available(dim):
+        c = tmp_path / "c.vtk"
+        assert Reference(dim).write_vtk(g, G, c) == 0
+        assert a.read_bytes() == c.read_bytes()
+    lines = a.read_text().splitlines()
+    assert lines[0] == "# vtk DataFile Version 3.0"
+    assert lines[4] == ("DIMENSIONS 9 7 5" if dim == 3 else "DIMENSIONS 9 7 1")
+    vals = np.array([float(x) for x in lines[10:]])
+    assert np.allclose(vals, G[g.interior].ravel(), rtol=1e-6, atol=0)
+
+
+def test_write_G_vtk_origin_and_errors(ls, tmp_path):
+    p = ls.default_params(3, 6)
+    p.x0, p.y0, p.z0 = -1.5, 0.25, 2.0
+    a = tmp_path / "o.vtk"
+    ls.write_G_vtk(p, np.zeros(p.shape), a)
+    assert a.read_text().splitlines()[5] == "ORIGIN -1.500000 0.250000 2.000000"
+    with pytest.raises(ls.LSError) as e:
+        ls.write_G_vtk(p, np.zeros(p.shape), tmp_path / "no_such_dir" / "x.vtk")
+    assert e.value.code == ls.LS_ERR_IO
+    with pytest.raises(ValueError):
+        ls.write_G_vtk(p, np.zeros((3, 3)), a)
+
+
 def test_write_G_io_error(ls, tmp_path):
     p = ls.default_params(2, 9)
     with pytest.raises(ls.LSError) as e:
