@@ -299,6 +299,7 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
     const int kb = ch * z.kz;
     const int ke = min(kb + z.kz, z.nz);
     const int64_t sz = z.sz;
+    const bool same = Gs == Gn;
     const double c2 = 0.25 * dt, c3 = (2.0 / 3.0) * dt;
 
     // Loads are unconditional: a position past the padded array reads the
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
         n.u = u[pl + col];
         n.v = v[pl + col];
         n.w = w[pl + col];
-        n.gn = Gn[pl + col];
+        n.gn = same ? 0.0 : Gn[pl + col];  // stage 1 passes G_n as the stencil field
         n.sl = sl ? sl[pl + col] : z.s_l;
 #pragma unroll
         for (int a = 0; a < NH; a++) n.halo[a] = Gs[pl + ohalo[a]];
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
         const int64_t plane = (int64_t)(k + 3) * sz;
         zG[6] = nx.lead;
         zt[4] = tsq(zG[4], zG[5], zG[6]);
-        const double cu = nx.u, cv = nx.v, cw = nx.w, cgn = nx.gn, csl = nx.sl;
+        const double cu = nx.u, cv = nx.v, cw = nx.w, cgn = same ? zG[3] : nx.gn, csl = nx.sl;
         if (k + 1 < ke) fetch(k + 1, nx);
         __syncthreads();  // plane k in sG
         // --- smoothness terms along x and y, effective velocity, orientation
