@@ -15,16 +15,17 @@
 //     the periodic fill (weno5_rk3.cpp:148-188).
 // Anchored-then-periodic (field.cpp:16-63) leaves: interior row 3 = the
 // initial field's row 3, interior row ny+2 = 2 row(ny+1) - row(ny), and every
-// ghost cell a doubly periodic image of the interior.  The stage kernels write
-// exactly that themselves: the thread of a row-(ny+2) cell also evaluates the
-// stage at rows ny and ny+1 of its column, and every interior cell writes its
-// own periodic images.  One launch per stage.
+// ghost cell a doubly periodic image of the interior.  The stage kernel
+// writes exactly that itself (k_sample_ymarch): a wavefront marching up a
+// strip has rows ny and ny+1 of its output when it reaches row ny+2, and
+// every interior cell writes its own periodic images.  One launch per stage.
 //
 // Arithmetic keeps the reference's evaluation order; the library is compiled
 // with -ffp-contract=off and IEEE division and sqrt, so results equal the
 // reference's CPU build bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -64,21 +65,13 @@ __device__ __forceinline__ double sface(double v0, double v1, double v2, double 
            w2 * (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
 }
 
-// dG_L and dG_R along stride s at P (sample/weno5_rk3.cpp:53-87).
-__device__ __forceinline__ void sderivs(const double* P, int64_t s, double h, double& dl, double& dr) {
-    const double fm = sface(P[-3 * s], P[-2 * s], P[-s], P[0], P[s]);      // face i-1/2
-    const double f0 = sface(P[-2 * s], P[-s], P[0], P[s], P[2 * s]);       // face i+1/2
-    const double fp = sface(P[-s], P[0], P[s], P[2 * s], P[3 * s]);        // face i+3/2
-    dl = (f0 - fm) / h;
-    dr = (fp - f0) / h;
-}
-
-// compute_geqn_rhs at one interior cell (sample/weno5_rk3.cpp:111-143).
-__device__ __forceinline__ double srhs(const SGeom& g, const double* __restrict__ G, int64_t c, double uc, double vc,
-                                       double S_L) {
-    double Gdxm, Gdxp, Gdym, Gdyp;
-    sderivs(G + c, 1, g.dx, Gdxm, Gdxp);
-    sderivs(G + c, g.nxt, g.dy, Gdym, Gdyp);
+// compute_geqn_rhs at one interior cell (sample/weno5_rk3.cpp:111-143) from
+// its three faces per axis, L at i-1/2, i+1/2, i+3/2 (dG_L and dG_R of
+// weno5_rk3.cpp:53-87 are their differences over h).
+__device__ __forceinline__ double srhs_f(const SGeom& g, double fmx, double f0x, double fpx, double fmy, double f0y,
+                                         double fpy, double uc, double vc, double S_L) {
+    const double Gdxm = (f0x - fmx) / g.dx, Gdxp = (fpx - f0x) / g.dx;
+    const double Gdym = (f0y - fmy) / g.dy, Gdyp = (fpy - f0y) / g.dy;
     const double Gdx_mean = 0.5 * (Gdxm + Gdxp);
     const double Gdy_mean = 0.5 * (Gdym + Gdyp);
     const double transport = uc * Gdx_mean + vc * Gdy_mean;
@@ -98,6 +91,17 @@ __device__ __forceinline__ double srhs(const SGeom& g, const double* __restrict_
     return -H;
 }
 
+// The same with the faces built for this cell alone.
+__device__ __forceinline__ double srhs(const SGeom& g, const double* __restrict__ G, int64_t c, double uc, double vc,
+                                       double S_L) {
+    const double* P = G + c;
+    const int64_t s = g.nxt;
+    return srhs_f(g, sface(P[-3], P[-2], P[-1], P[0], P[1]), sface(P[-2], P[-1], P[0], P[1], P[2]),
+                  sface(P[-1], P[0], P[1], P[2], P[3]), sface(P[-3 * s], P[-2 * s], P[-s], P[0], P[s]),
+                  sface(P[-2 * s], P[-s], P[0], P[s], P[2 * s]), sface(P[-s], P[0], P[s], P[2 * s], P[3 * s]), uc,
+                  vc, S_L);
+}
+
 // Stage combine at cell c; Gn_c is G_n at c.
 template <int STAGE>
 __device__ __forceinline__ double scombine(double gn, double gs, double r, double dt) {
@@ -113,45 +117,108 @@ __device__ __forceinline__ int s_img(int q, int n, int a) {
     return (a == 1 && q < 6) ? q + n : q - n;
 }
 
-// One RK3 stage over the interior plus the anchored rows and periodic ghosts.
-// out may alias Gn for stage 3 (each cell reads G_n only at itself; the
-// row-(ny+2) threads read rows ny and ny+1 of G_n from `keep`, which stage 2
-// filled).  Stage 2 writes G_n's rows ny and ny+1 into `keep`.
+// One RK3 stage over the interior plus the anchored rows and periodic ghosts,
+// as a y-march of one wavefront.  The 64 lanes of a strip of kSW = 62 cells
+// hold the x faces between interior columns xs-1+l and xs+l (l = 0..63), so
+// each x face is built once and read by the three cells that use it; each
+// lane also carries its column's 7 y-values and the y faces j-1/2, j+1/2 in
+// registers and builds one new y face per row.  A cell's arithmetic is
+// srhs_f's on the same face values as srhs, so the result is bitwise the
+// per-cell one.  Row 3 takes the initial field; row ny+2 (padded) is
+// 2 row(ny+1) - row(ny) of this stage's output, which the same wavefront has
+// just computed (the last chunk holds at least 3 rows).  out may alias Gn (stage 3):
+// a cell reads G_n only at itself, before writing it.
+constexpr int kSW = 62;
+#ifndef LS_SAMPLE_ROWS
+#define LS_SAMPLE_ROWS 4  // rows per chunk but the last
+#endif
+
+struct SLds {
+    double g[72];  // G of the current row at padded column xs + p
+    double f[64];  // lane l's x face
+};
+
 template <int STAGE>
-__global__ __launch_bounds__(256) void k_sample_stage(SGeom g, const double* __restrict__ Gs, const double* Gn,
-                                                      double* out, const double* __restrict__ u,
+__global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int nch, const double* __restrict__ Gs,
+                                                      const double* Gn, double* out, const double* __restrict__ u,
                                                       const double* __restrict__ v,
-                                                      const double* __restrict__ init, double* keep,
-                                                      double S_L, double dt) {
-    const int i = 3 + blockIdx.x * kSX + threadIdx.x;
-    const int j = 3 + blockIdx.y * kSY + threadIdx.y;
-    if (i > g.nx + 2 || j > g.ny + 2) return;
-    const int64_t c = (int64_t)j * g.nxt + i;
-    double val;
-    if (j == 3) {
-        val = init[c];
-    } else if (j == g.ny + 2) {
-        // 2 row(ny+1) - row(ny) of this stage's output
-        const int64_t c1 = c - g.nxt, c0 = c - 2 * g.nxt;
-        double gn1, gn0;
-        if (STAGE == 3) {
-            gn1 = keep[g.nxt + i];
-            gn0 = keep[i];
-        } else {
-            gn1 = Gn[c1];
-            gn0 = Gn[c0];
-        }
-        const double o1 = scombine<STAGE>(gn1, Gs[c1], srhs(g, Gs, c1, u[c1], v[c1], S_L), dt);
-        const double o0 = scombine<STAGE>(gn0, Gs[c0], srhs(g, Gs, c0, u[c0], v[c0], S_L), dt);
-        val = 2.0 * o1 - o0;
-    } else {
-        const double gn = Gn[c];
-        if (STAGE == 2 && j >= g.ny) keep[(j - g.ny) * g.nxt + i] = gn;
-        val = scombine<STAGE>(gn, Gs[c], srhs(g, Gs, c, u[c], v[c], S_L), dt);
+                                                      const double* __restrict__ init, double S_L, double dt) {
+    __shared__ SLds L;
+    const int t = (int)blockIdx.x;
+    const int sidx = t % strips, ch = t / strips;
+    const int l = (int)threadIdx.x;
+    const int xs = sidx * kSW;               // first interior column of the strip
+    const int W = min(kSW, g.nx - xs);
+    const int X = xs + 2 + l;                // padded column of lane l
+    const bool col = X <= g.nx + 5;
+    const bool cell = l >= 1 && l <= W;
+    // chunks of LS_SAMPLE_ROWS rows; the last one takes the rest (3 rows or more)
+    const int r0 = 3 + ch * LS_SAMPLE_ROWS, r1 = ch + 1 == nch ? g.ny + 3 : r0 + LS_SAMPLE_ROWS;
+    const int64_t s = g.nxt;
+    const bool same = Gs == Gn;
+    // LDS columns outside the lanes' own: p = 0, 1 (lanes 0, 1) and 66, 67 (lanes 2, 3)
+    const bool xl = l < 4;
+    const int xp = l < 2 ? l : 64 + l;
+    const bool xok = xs + xp <= g.nx + 5;
+    double ring[7];  // G(X, r-3 .. r+3)
+#pragma unroll
+    for (int q = 0; q < 7; q++) ring[q] = col ? Gs[(int64_t)(r0 - 3 + q) * s + X] : 0.0;
+    double fy0 = sface(ring[0], ring[1], ring[2], ring[3], ring[4]);  // y face r-1/2
+    double fy1 = sface(ring[1], ring[2], ring[3], ring[4], ring[5]);  // y face r+1/2
+    double cu = 0.0, cv = 0.0, cgn = 0.0, xh = 0.0;
+    if (cell) {
+        const int64_t c = (int64_t)r0 * s + X;
+        cu = u[c];
+        cv = v[c];
+        if (!same) cgn = Gn[c];
     }
-    const int nj = s_nimg(j, g.ny), ni = s_nimg(i, g.nx);
-    for (int b = 0; b < nj; b++)
-        for (int a = 0; a < ni; a++) out[(int64_t)s_img(j, g.ny, b) * g.nxt + s_img(i, g.nx, a)] = val;
+    if (xl && xok) xh = Gs[(int64_t)r0 * s + xs + xp];
+    double o1 = 0.0, o0 = 0.0;  // this stage's output at rows r-1, r-2
+    for (int r = r0; r < r1; r++) {
+        L.g[l + 2] = ring[3];
+        if (xl) L.g[xp] = xh;
+        double nr = 0.0, nu = 0.0, nv = 0.0, ngn = 0.0, nxh = 0.0;
+        if (r + 1 < r1) {
+            if (col) nr = Gs[(int64_t)(r + 4) * s + X];
+            if (cell) {
+                const int64_t c1 = (int64_t)(r + 1) * s + X;
+                nu = u[c1];
+                nv = v[c1];
+                if (!same) ngn = Gn[c1];
+            }
+            if (xl && xok) nxh = Gs[(int64_t)(r + 1) * s + xs + xp];
+        }
+        __syncthreads();  // row in LDS
+        L.f[l] = sface(L.g[l], L.g[l + 1], L.g[l + 2], L.g[l + 3], L.g[l + 4]);
+        const double fy2 = sface(ring[2], ring[3], ring[4], ring[5], ring[6]);  // y face r+3/2
+        __syncthreads();  // x faces
+        if (cell) {
+            const int64_t c = (int64_t)r * s + X;
+            double val;
+            if (r == 3) {
+                val = init[c];
+            } else if (r == g.ny + 2) {
+                val = 2.0 * o1 - o0;
+            } else {
+                const double rr = srhs_f(g, L.f[l - 1], L.f[l], L.f[l + 1], fy0, fy1, fy2, cu, cv, S_L);
+                val = scombine<STAGE>(same ? ring[3] : cgn, ring[3], rr, dt);
+            }
+            const int nj = s_nimg(r, g.ny), ni = s_nimg(X, g.nx);
+            for (int b = 0; b < nj; b++)
+                for (int a = 0; a < ni; a++) out[(int64_t)s_img(r, g.ny, b) * s + s_img(X, g.nx, a)] = val;
+            o0 = o1;
+            o1 = val;
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) ring[q] = ring[q + 1];
+        ring[6] = nr;
+        fy0 = fy1;
+        fy1 = fy2;
+        cu = nu;
+        cv = nv;
+        cgn = ngn;
+        xh = nxh;
+    }
 }
 
 // compute_geqn_rhs into a padded array (zero outside the interior).
@@ -315,7 +382,6 @@ struct ls_sample {
     int64_t n;
     double* G1 = nullptr;
     double* G2 = nullptr;
-    double* keep = nullptr;  // rows ny and ny+1 of G_n for the in-place stage 3
 };
 
 namespace {
@@ -329,10 +395,12 @@ int launch_ok(const char* what) {
 int rk3(ls_sample* s, double* G, const double* u, const double* v, double dt, double S_L, const double* init,
         hipStream_t st) {
     const SGeom& g = s->g;
-    const dim3 grid = interior_grid(g, 0), blk(kSX, kSY, 1);
-    hipLaunchKernelGGL(k_sample_stage<1>, grid, blk, 0, st, g, G, G, s->G1, u, v, init, s->keep, S_L, dt);
-    hipLaunchKernelGGL(k_sample_stage<2>, grid, blk, 0, st, g, s->G1, G, s->G2, u, v, init, s->keep, S_L, dt);
-    hipLaunchKernelGGL(k_sample_stage<3>, grid, blk, 0, st, g, s->G2, G, G, u, v, init, s->keep, S_L, dt);
+    const int strips = (g.nx + kSW - 1) / kSW;
+    const int nch = (g.ny - 3) / LS_SAMPLE_ROWS + 1;  // the last chunk starts at row ny or below
+    const dim3 grid((unsigned)(strips * nch)), blk(64, 1, 1);
+    hipLaunchKernelGGL(k_sample_ymarch<1>, grid, blk, 0, st, g, strips, nch, G, G, s->G1, u, v, init, S_L, dt);
+    hipLaunchKernelGGL(k_sample_ymarch<2>, grid, blk, 0, st, g, strips, nch, s->G1, G, s->G2, u, v, init, S_L, dt);
+    hipLaunchKernelGGL(k_sample_ymarch<3>, grid, blk, 0, st, g, strips, nch, s->G2, G, G, u, v, init, S_L, dt);
     return launch_ok("sample stage launch failed");
 }
 
@@ -372,11 +440,9 @@ int ls_sample_create(int32_t nx, int32_t ny, double dx, double dy, ls_sample_t**
     s->g.dy = dy;
     s->n = (int64_t)s->g.nxt * s->g.nyt;
     const size_t fb = (size_t)s->n * sizeof(double);
-    if (hipMalloc((void**)&s->G1, fb) != hipSuccess || hipMalloc((void**)&s->G2, fb) != hipSuccess ||
-        hipMalloc((void**)&s->keep, 2 * (size_t)s->g.nxt * sizeof(double)) != hipSuccess) {
+    if (hipMalloc((void**)&s->G1, fb) != hipSuccess || hipMalloc((void**)&s->G2, fb) != hipSuccess) {
         (void)hipFree(s->G1);
         (void)hipFree(s->G2);
-        (void)hipFree(s->keep);
         delete s;
         return lsi::set_error(LS_ERR_NOMEM, "sample workspace allocation failed");
     }
@@ -386,7 +452,6 @@ int ls_sample_create(int32_t nx, int32_t ny, double dx, double dy, ls_sample_t**
         hipDeviceSynchronize() != hipSuccess) {
         (void)hipFree(s->G1);
         (void)hipFree(s->G2);
-        (void)hipFree(s->keep);
         delete s;
         return lsi::set_error(LS_ERR_DEVICE, "sample workspace initialisation failed");
     }
@@ -398,7 +463,6 @@ int ls_sample_destroy(ls_sample_t* s) {
     if (!s) return LS_OK;
     (void)hipFree(s->G1);
     (void)hipFree(s->G2);
-    (void)hipFree(s->keep);
     delete s;
     return LS_OK;
 }
