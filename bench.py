@@ -324,7 +324,7 @@ def bench_sample(ls, steps, n=800):
            "ms_per_step": wall / steps * 1e3,
            "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": ach / HBM_PEAK, "traffic": None,
-                        "kernel": "k_sample_stage<1..3> (3 launches per RK3 step)",
+                        "kernel": "k_sample_ymarch<1..3> (3 launches per RK3 step)",
                         "rk3_kernels_ms": t_step * 1e3, "algorithmic_bytes_per_cell_step": bytes_step}}
     exe = os.path.join(ROOT, "oracle", "_ref", "sample_bench")
     if os.path.exists(exe):
