@@ -257,9 +257,8 @@ int ls_batch_fill_ghosts(ls_batch_t* b, double* G, void* stream);
  * captures and replays a hipGraph.  Does not synchronise. */
 int ls_batch_advance(ls_batch_t* b, double* G, const double* u, const double* v, int32_t nsteps,
                      int32_t use_graph, void* stream);
-/* Schedule shapes 0..3.  Exact mode: one thread per cell in tiles of 64x4,
- * 64x2, 128x2, 32x8 threads.  Fast mode: the y-march, one wavefront per
- * 64-column strip and chunk of 32, 16, 4, 8 rows.  ls_batch_autotune times
+/* Schedule shapes 0..3: the y-march, one wavefront per 64-column strip and
+ * chunk of 32, 16, 4 or 8 rows (both modes).  ls_batch_autotune times
  * probe_steps steps of each (two passes) on a scratch copy and keeps the
  * fastest (*best_ms = its time per step).  The result does not depend on the
  * shape.  Synchronises the stream. */

@@ -2,13 +2,14 @@
 //
 // All members' padded fields live in one packed device buffer (member m at
 // element offset off_m, 256-byte aligned).  One kernel launch per RK3 stage
-// covers every member: blocks walk a global tile list (member, tile x, tile y)
-// in a grid-stride loop, so small and large grids share the machine instead of
-// running one after another.  The stage kernel writes each output's ghost
-// cells itself (periodic / zero-gradient images), so a time step is three
-// launches.  ls_batch_advance captures the whole multi-step sequence into one
-// hipGraph and replays it, and ls_batch_autotune picks the tile shape that
-// runs fastest for the batch at hand.
+// covers every member: the y-march of ls_ymarch.hpp, one wavefront per unit
+// (member, 64-column strip, chunk of rows), units dealt XCD-contiguously, so
+// small and large grids share the machine instead of running one after
+// another.  The stage kernel writes each output's ghost cells itself
+// (periodic / zero-gradient images), so a time step is three launches.
+// ls_batch_advance captures the whole multi-step sequence into one hipGraph
+// and replays it, and ls_batch_autotune picks the chunk length that runs
+// fastest for the batch at hand.
 //
 // Per member the arithmetic is the single-grid solver's: the exact mode is
 // bitwise equal to the oracle, the fast mode within rounding (the same device
@@ -43,56 +44,14 @@ struct BTile {
     int32_t member, tx, ty, pad;
 };
 
-struct TileShape {
-    int tx, ty;
-};
-// Exact mode: one thread per cell, tiles of tx x ty threads.  Fast mode: the
-// y-march (ls_ymarch.hpp), one wavefront per 64-column strip and chunk of
-// kChunks[shape] rows.
-constexpr TileShape kShapes[] = {{64, 4}, {64, 2}, {128, 2}, {32, 8}};
+// Schedule shapes: y-march chunks of kChunks[shape] rows (ls_ymarch.hpp), one
+// wavefront per 64-column strip and chunk.
 constexpr int kChunks[] = {32, 16, 4, 8};
 constexpr int kNumShapes = 4;
 
-template <int STAGE, int TX, int TY>
-__global__ __launch_bounds__(TX* TY) void k_batch_stage(const BMember* __restrict__ mem,
-                                                        const BTile* __restrict__ tiles, int ntiles, int ch,
-                                                        const double* __restrict__ Gs,
-                                                        const double* __restrict__ Gn,
-                                                        double* __restrict__ out,
-                                                        const double* __restrict__ u,
-                                                        const double* __restrict__ v) {
-    (void)ch;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const BTile tl = tiles[t];
-        const BMember m = mem[tl.member];
-        const int i = tl.tx * TX + (int)threadIdx.x;
-        const int j = tl.ty * TY + (int)threadIdx.y;
-        if (i >= m.nx || j >= m.ny) continue;
-        const int64_t sy = m.nxt;
-        const int64_t c = m.off + (int64_t)(j + 3) * sy + (i + 3);
-        const double* P = Gs + c;
-        lsx::Geom g;
-        g.sy = sy;
-        g.sz = 0;
-        g.dx = m.dx;
-        g.dy = m.dy;
-        g.eps = m.eps;
-        const double r = lsx::rhs_cell<2>(g, P, u[c], v[c], 0.0, m.s_l);
-        const double o = lsx::stage_combine<STAGE>(Gn[c], P[0], r, m.dt);
-        out[c] = o;
-        if (i < 3 || i >= m.nx - 3 || j < 3 || j >= m.ny - 3) {
-            const int nxi = lsf::n_images(i, m.nx, m.bc0), nyi = lsf::n_images(j, m.ny, m.bc1);
-            for (int b = 0; b < nyi; b++) {
-                const int64_t pj = m.off + (int64_t)lsf::image(j, m.ny, m.bc1, b) * sy;
-                for (int e = 0; e < nxi; e++)
-                    if (b | e) out[pj + lsf::image(i, m.nx, m.bc0, e)] = o;
-            }
-        }
-    }
-}
-
-// Fast mode: units (member, strip, chunk of ch rows), one wavefront each.
-template <int STAGE>
+// Units (member, strip, chunk of ch rows), one wavefront each; fast or exact
+// arithmetic.
+template <int STAGE, bool EXACT>
 __global__ __launch_bounds__(lsf::kYW) void k_batch_ymarch(const BMember* __restrict__ mem,
                                                            const BTile* __restrict__ tiles, int ntiles, int ch,
                                                            const double* __restrict__ Gs,
@@ -125,8 +84,10 @@ __global__ __launch_bounds__(lsf::kYW) void k_batch_ymarch(const BMember* __rest
         m.inv2dy = b.inv2dy;
         m.inv6dx = b.inv6dx;
         m.inv6dy = b.inv6dy;
+        m.dx = b.dx;
+        m.dy = b.dy;
         const int y0 = tl.ty * ch;
-        lsf::ymarch_strip<STAGE>(m, tl.tx * lsf::kYW, y0, min(y0 + ch, b.ny), Gs, Gn, out, u, v, nullptr, L);
+        lsf::ymarch_strip<STAGE, EXACT>(m, tl.tx * lsf::kYW, y0, min(y0 + ch, b.ny), Gs, Gn, out, u, v, nullptr, L);
     }
 }
 
@@ -155,25 +116,15 @@ __global__ __launch_bounds__(256) void k_copy(const double* __restrict__ a, doub
 typedef void (*StageFn)(const BMember*, const BTile*, int, int, const double*, const double*, double*,
                         const double*, const double*);
 
-template <int TX, int TY>
-void pick3(StageFn* f) {
-    f[0] = k_batch_stage<1, TX, TY>;
-    f[1] = k_batch_stage<2, TX, TY>;
-    f[2] = k_batch_stage<3, TX, TY>;
-}
-
-void stage_fns(int mode, int shape, StageFn* f) {
+void stage_fns(int mode, StageFn* f) {
     if (mode == LS_MODE_FAST) {
-        f[0] = k_batch_ymarch<1>;
-        f[1] = k_batch_ymarch<2>;
-        f[2] = k_batch_ymarch<3>;
-        return;
-    }
-    switch (shape) {
-        case 0: pick3<64, 4>(f); break;
-        case 1: pick3<64, 2>(f); break;
-        case 2: pick3<128, 2>(f); break;
-        default: pick3<32, 8>(f); break;
+        f[0] = k_batch_ymarch<1, false>;
+        f[1] = k_batch_ymarch<2, false>;
+        f[2] = k_batch_ymarch<3, false>;
+    } else {
+        f[0] = k_batch_ymarch<1, true>;
+        f[1] = k_batch_ymarch<2, true>;
+        f[2] = k_batch_ymarch<3, true>;
     }
 }
 
@@ -210,13 +161,11 @@ static void drop_graph(ls_batch* b) {
 // Enqueue nsteps RK3 steps on `st` (no synchronisation).
 static int enqueue_steps(ls_batch* b, double* G, const double* u, const double* v, int nsteps, hipStream_t st) {
     StageFn f[3];
-    stage_fns(b->mode, b->shape, f);
-    const bool fast = b->mode == LS_MODE_FAST;
-    const TileShape sh = kShapes[b->shape];
+    stage_fns(b->mode, f);
     const int nt = b->ntiles[b->shape];
     const int ch = kChunks[b->shape];
-    const unsigned grid = fast ? (unsigned)((nt + 7) / 8 * 8) : (unsigned)std::min(nt, 256 * 16);
-    const dim3 block(fast ? lsf::kYW : sh.tx, fast ? 1 : sh.ty, 1);
+    const unsigned grid = (unsigned)((nt + 7) / 8 * 8);
+    const dim3 block(lsf::kYW, 1, 1);
     double* cur = G;
     double* a = b->buf[0];
     double* c = b->buf[1];
@@ -290,8 +239,8 @@ int ls_batch_create(const ls_batch_member_t* members, int32_t count, int32_t mod
         std::sort(order.begin(), order.end(), [&](int x, int y) {
             return (int64_t)members[x].nx * members[x].ny > (int64_t)members[y].nx * members[y].ny;
         });
-        const int ux = mode == LS_MODE_FAST ? lsf::kYW : kShapes[s].tx;
-        const int uy = mode == LS_MODE_FAST ? kChunks[s] : kShapes[s].ty;
+        const int ux = lsf::kYW;
+        const int uy = kChunks[s];
         for (int m : order) {
             const int tx = (members[m].nx + ux - 1) / ux;
             const int ty = (members[m].ny + uy - 1) / uy;

@@ -58,7 +58,7 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 
 // 2D stage: one wavefront per (64-column strip, chunk of rows), y-marching
 // (ls_ymarch.hpp).  Units are dealt XCD-contiguously, strips fastest.
-template <int STAGE>
+template <int STAGE, bool EXACT>
 __global__ __launch_bounds__(kYW) void k_stage_ymarch(YGrid m, int strips, int ch, int64_t units, int64_t nblocks,
                                                       const double* __restrict__ Gs, const double* __restrict__ Gn,
                                                       double* __restrict__ out, const double* __restrict__ u,
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kYW) void k_stage_ymarch(YGrid m, int strips, int c
     if (t >= units) return;
     const int sx = (int)(t % strips), cy = (int)(t / strips);
     const int y0 = cy * ch;
-    ymarch_strip<STAGE>(m, sx * kYW, y0, min(y0 + ch, m.ny), Gs, Gn, out, u, v, sl, L);
+    ymarch_strip<STAGE, EXACT>(m, sx * kYW, y0, min(y0 + ch, m.ny), Gs, Gn, out, u, v, sl, L);
 }
 
 // 2D stage for small grids: one thread per cell, 64x4 tiles, the y-march's
@@ -889,11 +889,7 @@ int ymarch_chunk(int nx, int ny) {
     return strips * ((ny + ch - 1) / ch) < 2048 ? 0 : ch;
 }
 
-template <int DIM>
-int launch_stage_fast(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
-                      const double* u, const double* v, const double* w, const double* sl, double dt,
-                      hipStream_t st) {
-    if (DIM == 3) return launch_stage_zmarch(g, stage, Gs, Gn, out, u, v, w, sl, dt, st, false);
+static YGrid make_y(const Geom& g, double dt) {
     const FastGeom f = make_fast(g);
     YGrid m;
     m.nx = g.nx;
@@ -911,30 +907,58 @@ int launch_stage_fast(const Geom& g, int stage, const double* Gs, const double* 
     m.inv2dy = f.inv2dy;
     m.inv6dx = f.inv6dx;
     m.inv6dy = f.inv6dy;
+    m.dx = g.dx;
+    m.dy = g.dy;
+    return m;
+}
+
+bool ymarch_fits(const Geom& g) { return g.dim == 2 && ymarch_chunk(g.nx, g.ny) != 0; }
+
+// The 2D stage as a y-march (the grid must fit it).  Returns 1 when the
+// kernel also filled the ghost cells of out.
+int launch_stage_ymarch(const Geom& g, int stage, const double* Gs, const double* Gn, double* out, const double* u,
+                        const double* v, const double* sl, double dt, hipStream_t st, bool exact) {
+    const YGrid m = make_y(g, dt);
     const int ch = ymarch_chunk(g.nx, g.ny);
-    if (ch == 0) {
-        const int64_t tiles = (int64_t)f.tiles_x * f.tiles_y;
-        if (f.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
-        const dim3 grid((unsigned)f.nblocks), block(kFX, kFY, 1);
-        if (stage == 1)
-            hipLaunchKernelGGL((k_stage_ycell<1>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
-        else if (stage == 2)
-            hipLaunchKernelGGL((k_stage_ycell<2>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
-        else
-            hipLaunchKernelGGL((k_stage_ycell<3>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
+    const int strips = (g.nx + kYW - 1) / kYW;
+    const int64_t units = (int64_t)strips * ((g.ny + ch - 1) / ch);
+    const int64_t nblocks = ((units + 7) / 8) * 8;
+    if (ch == 0 || nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid does not fit the y-march");
+    const dim3 grid((unsigned)nblocks), block(kYW, 1, 1);
+#define LS_YMARCH_LAUNCH(E)                                                                                      \
+    if (stage == 1)                                                                                              \
+        hipLaunchKernelGGL((k_stage_ymarch<1, E>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl); \
+    else if (stage == 2)                                                                                         \
+        hipLaunchKernelGGL((k_stage_ymarch<2, E>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl); \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_stage_ymarch<3, E>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl);
+    if (exact) {
+        LS_YMARCH_LAUNCH(true)
     } else {
-        const int strips = (g.nx + kYW - 1) / kYW;
-        const int64_t units = (int64_t)strips * ((g.ny + ch - 1) / ch);
-        const int64_t nblocks = ((units + 7) / 8) * 8;
-        if (nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
-        const dim3 grid((unsigned)nblocks), block(kYW, 1, 1);
-        if (stage == 1)
-            hipLaunchKernelGGL((k_stage_ymarch<1>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl);
-        else if (stage == 2)
-            hipLaunchKernelGGL((k_stage_ymarch<2>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl);
-        else
-            hipLaunchKernelGGL((k_stage_ymarch<3>), grid, block, 0, st, m, strips, ch, units, nblocks, Gs, Gn, out, u, v, sl);
+        LS_YMARCH_LAUNCH(false)
     }
+#undef LS_YMARCH_LAUNCH
+    if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "y-march stage launch failed");
+    return m.fuse_ghosts ? 1 : 0;
+}
+
+template <int DIM>
+int launch_stage_fast(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
+                      const double* u, const double* v, const double* w, const double* sl, double dt,
+                      hipStream_t st) {
+    if (DIM == 3) return launch_stage_zmarch(g, stage, Gs, Gn, out, u, v, w, sl, dt, st, false);
+    if (ymarch_fits(g)) return launch_stage_ymarch(g, stage, Gs, Gn, out, u, v, sl, dt, st, false);
+    const FastGeom f = make_fast(g);
+    const YGrid m = make_y(g, dt);
+    const int64_t tiles = (int64_t)f.tiles_x * f.tiles_y;
+    if (f.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
+    const dim3 grid((unsigned)f.nblocks), block(kFX, kFY, 1);
+    if (stage == 1)
+        hipLaunchKernelGGL((k_stage_ycell<1>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
+    else if (stage == 2)
+        hipLaunchKernelGGL((k_stage_ycell<2>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
+    else
+        hipLaunchKernelGGL((k_stage_ycell<3>), grid, block, 0, st, m, f.tiles_x, tiles, f.nblocks, Gs, Gn, out, u, v, sl);
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast stage launch failed");
     return m.fuse_ghosts ? 1 : 0;
 }

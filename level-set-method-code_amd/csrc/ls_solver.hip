@@ -337,6 +337,7 @@ static int launch_stage(const ls_solver* s, int stage, const double* Gs, const d
     if constexpr (DIM == 3) {
         return lsf::launch_stage_exact3d(g, stage, Gs, Gn, out, u, v, w, sl, dt, st);
     } else {
+        if (lsf::ymarch_fits(g)) return lsf::launch_stage_ymarch(g, stage, Gs, Gn, out, u, v, sl, dt, st, true);
         dim3 grid = cell_grid(g), block(kTX, kTY, 1);
         if (stage == 1)
             hipLaunchKernelGGL((k_stage_exact<DIM, 1>), grid, block, 0, st, g, Gs, Gn, out, u, v, w, sl, dt);

@@ -10,13 +10,16 @@
 // shared between them: x faces through LDS, the y face carried in registers to
 // the next row, which rebuilds it only where its orientation differs.  Both
 // axes go over one denominator, so a cell costs one refined reciprocal.  The
-// arithmetic is the fast mode's (ls_fast_math.hpp): the result does not
-// depend on the strip or chunk partition, since every face is a pure function
-// of its stencil.
+// arithmetic is the fast mode's (ls_fast_math.hpp), or with EXACT the
+// reference's (weno5_left / weno5_right values, its divisions and expression
+// order, bitwise equal to the oracle).  Either way the result does not depend
+// on the strip or chunk partition, since every face is a pure function of
+// its stencil.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include "ls_api.h"
+#include "ls_exact.hpp"
 #include "ls_fast_math.hpp"
 
 namespace lsf {
@@ -45,6 +48,7 @@ struct YGrid {
     int64_t sy, off;      // padded row pitch; element offset of padded (0, 0)
     double s_l, eps, dt;
     double inv2dx, inv2dy, inv6dx, inv6dy;
+    double dx, dy;        // exact mode divides by these, as the reference does
 };
 
 // Rows the y-march loads ahead of the one it computes (1 or 2).
@@ -155,7 +159,7 @@ __device__ __forceinline__ void ycell(const YGrid& m, int i, int j, const double
 
 // Rows [y0, y1) of the strip that starts at column x0.  Called by every lane
 // of a one-wavefront block (blockDim.x == 64) with the same arguments.
-template <int STAGE>
+template <int STAGE, bool EXACT = false>
 __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int y1,
                                              const double* __restrict__ Gs, const double* __restrict__ Gn,
                                              double* __restrict__ out, const double* __restrict__ u,
@@ -177,9 +181,9 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
     double r[7];  // G(i, j-3 .. j+3)
 #pragma unroll
     for (int q = 0; q < 7; q++) r[q] = act ? Gs[col + prow(y0 - 3 + q)] : 0.0;
-    double ty[5];  // T(i, j-2 .. j+2) along y
+    double ty[5];  // T(i, j-2 .. j+2) along y (fast mode)
 #pragma unroll
-    for (int q = 0; q < 5; q++) ty[q] = tsq(r[q], r[q + 1], r[q + 2]);
+    for (int q = 0; q < 5; q++) ty[q] = EXACT ? 0.0 : tsq(r[q], r[q + 1], r[q + 2]);
     // point data of row jr and the ring's next value G(i, jr + 3)
     auto fetch = [&](int jr, bool with_g) {
         YRow q;
@@ -211,13 +215,26 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         const YRow fresh = fetch(j + LS_YMARCH_PF, true);
         __syncthreads();  // row
         // --- smoothness terms along x, u_eff, orientations
-        if (act) L.t[3 + l] = tsq(L.g[2 + l], L.g[3 + l], L.g[4 + l]);
-        if (l < 4) {  // T at x0-2, x0-1, xe, xe+1
-            const int p = l < 2 ? 1 + l : xe - x0 + 1 + l;
-            L.t[p] = tsq(L.g[p - 1], L.g[p], L.g[p + 1]);
+        if (!EXACT) {
+            if (act) L.t[3 + l] = tsq(L.g[2 + l], L.g[3 + l], L.g[4 + l]);
+            if (l < 4) {  // T at x0-2, x0-1, xe, xe+1
+                const int p = l < 2 ? 1 + l : xe - x0 + 1 + l;
+                L.t[p] = tsq(L.g[p - 1], L.g[p], L.g[p + 1]);
+            }
         }
         double ue = cur.u, ve = cur.v;
-        if (act) yueff(m, L.g[2 + l], L.g[4 + l], r[2], r[4], cur.sl, ue, ve);
+        if (act && EXACT) {  // computeRHS (rk3.h:54-98), the reference's expression order
+            double gx = 0.0, gy = 0.0, gm = 1.0;
+            if (cur.sl > m.eps) {
+                gx = (L.g[4 + l] - L.g[2 + l]) / (2.0 * m.dx);
+                gy = (r[4] - r[2]) / (2.0 * m.dy);
+                gm = sqrt(gx * gx + gy * gy + m.eps);
+            }
+            ue = cur.u - cur.sl * gx / gm;
+            ve = cur.v - cur.sl * gy / gm;
+        } else if (act) {
+            yueff(m, L.g[2 + l], L.g[4 + l], r[2], r[4], cur.sl, ue, ve);
+        }
         const bool Lx = ue >= 0.0, Ly = ve >= 0.0;
         L.ori[l] = (unsigned char)Lx;
         __syncthreads();  // T, orientations
@@ -232,29 +249,51 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
                 nL = nL || o;
                 nR = nR || !o;
             }
-            face_put<1, 1, kYW + 1>(&L.g[3 + l], &L.t[3 + l], nL, nR, &L.f[0][l + 1]);
-            // y face j-1/2 in this cell's orientation (carried unless it differs), and j+1/2
-            if (ylo_L != (int)Ly) {
-                if (Ly) ylo = face_core<false>(r[0], r[1], r[2], r[3], r[4], ty[0], ty[1], ty[2]);
-                else ylo = face_core<true>(r[5], r[4], r[3], r[2], r[1], ty[3], ty[2], ty[1]);
+            if (EXACT) {  // weno5_left / weno5_right values (weno5.h:62-122); .x holds the value
+                const double* g = &L.g[3 + l];
+                if (nL) L.f[0][l + 1].x = lsx::weno_l(g[-2], g[-1], g[0], g[1], g[2]);
+                if (nR) L.f[1][l + 1].x = lsx::weno_r(g[-1], g[0], g[1], g[2], g[3]);
+                if (ylo_L != (int)Ly)
+                    ylo.n = Ly ? lsx::weno_l(r[0], r[1], r[2], r[3], r[4]) : lsx::weno_r(r[1], r[2], r[3], r[4], r[5]);
+                yhi.n = Ly ? lsx::weno_l(r[1], r[2], r[3], r[4], r[5]) : lsx::weno_r(r[2], r[3], r[4], r[5], r[6]);
+            } else {
+                face_put<1, 1, kYW + 1>(&L.g[3 + l], &L.t[3 + l], nL, nR, &L.f[0][l + 1]);
+                // y face j-1/2 in this cell's orientation (carried unless it differs), and j+1/2
+                if (ylo_L != (int)Ly) {
+                    if (Ly) ylo = face_core<false>(r[0], r[1], r[2], r[3], r[4], ty[0], ty[1], ty[2]);
+                    else ylo = face_core<true>(r[5], r[4], r[3], r[2], r[1], ty[3], ty[2], ty[1]);
+                }
+                if (Ly) yhi = face_core<false>(r[1], r[2], r[3], r[4], r[5], ty[1], ty[2], ty[3]);
+                else yhi = face_core<true>(r[6], r[5], r[4], r[3], r[2], ty[4], ty[3], ty[2]);
             }
-            if (Ly) yhi = face_core<false>(r[1], r[2], r[3], r[4], r[5], ty[1], ty[2], ty[3]);
-            else yhi = face_core<true>(r[6], r[5], r[4], r[3], r[2], ty[4], ty[3], ty[2]);
         }
-        if (l == 0)  // x face x0-1/2 in the orientation of the strip's first cell
-            face_put<1, 1, kYW + 1>(&L.g[2], &L.t[2], Lx, !Lx, &L.f[0][0]);
+        if (l == 0) {  // x face x0-1/2 in the orientation of the strip's first cell
+            if (EXACT)
+                L.f[Lx ? 0 : 1][0].x = Lx ? lsx::weno_l(L.g[0], L.g[1], L.g[2], L.g[3], L.g[4])
+                                          : lsx::weno_r(L.g[1], L.g[2], L.g[3], L.g[4], L.g[5]);
+            else
+                face_put<1, 1, kYW + 1>(&L.g[2], &L.t[2], Lx, !Lx, &L.f[0][0]);
+        }
         __syncthreads();  // faces
         // --- derivatives over one denominator, RHS, stage combine
         if (act) {
             const int o = Lx ? 0 : 1;
             const double2 fl = L.f[o][l], fh = L.f[o][l + 1];
-            Face xl, xh;
-            xl.n = fl.x;
-            xl.d = fl.y;
-            xh.n = fh.x;
-            xh.d = fh.y;
-            const double rr = yrhs(m, xl, xh, ylo, yhi, ue, ve);
-            const double o2 = ystage<STAGE>(same ? r[3] : cur.gn, r[3], rr, m.dt);
+            double o2;
+            if (EXACT) {  // (F(i+1/2) - F(i-1/2)) / h, rhs and the stage in the reference's order
+                const double dgx = (fh.x - fl.x) / m.dx;
+                const double dgy = (yhi.n - ylo.n) / m.dy;
+                const double rr = -(ue * dgx + ve * dgy);
+                o2 = lsx::stage_combine<STAGE>(same ? r[3] : cur.gn, r[3], rr, m.dt);
+            } else {  // both axes over one denominator
+                Face xl, xh;
+                xl.n = fl.x;
+                xl.d = fl.y;
+                xh.n = fh.x;
+                xh.d = fh.y;
+                const double rr = yrhs(m, xl, xh, ylo, yhi, ue, ve);
+                o2 = ystage<STAGE>(same ? r[3] : cur.gn, r[3], rr, m.dt);
+            }
             out[col + prow(j)] = o2;
             // ghost cells of out: written by the interior cell they copy
             ystore_images(m, out, i, j, o2);
@@ -272,7 +311,7 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         r[6] = cur.g;
 #pragma unroll
         for (int q = 0; q < 4; q++) ty[q] = ty[q + 1];
-        ty[4] = tsq(r[4], r[5], r[6]);
+        ty[4] = EXACT ? 0.0 : tsq(r[4], r[5], r[6]);
     }
 }
 

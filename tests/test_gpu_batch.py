@@ -80,6 +80,21 @@ def test_batch_graph_replays(ls, cuda):
         assert cuda.equal(eager, graphed), "replay %d" % rep
 
 
+def test_batch_exact_all_shapes_bitwise(ls, oracle, cuda):
+    """Exact mode on every chunk length: bitwise the oracle each time."""
+    b, G, U, V, inputs = _build(ls, cuda, ls.MODE_EXACT)
+    steps = 5
+    refs = [oracle.advance(g, Gh, uh, vh, dt=dt, t_final=1e30, max_steps=steps)[0]
+            for (g, Gh, uh, vh, dt) in inputs]
+    for shape in range(4):
+        Gs = G.clone()
+        b.set_tile(shape)
+        b.advance(Gs, U, V, steps, graph=False)
+        cuda.cuda.synchronize()
+        for q in range(len(inputs)):
+            assert np.array_equal(b.view(Gs, q).cpu().numpy(), refs[q]), "shape %d member %d" % (shape, q)
+
+
 def test_batch_autotune_keeps_results(ls, cuda):
     b, G, U, V, _ = _build(ls, cuda, ls.MODE_EXACT)
     G2 = G.clone()
@@ -89,7 +104,7 @@ def test_batch_autotune_keeps_results(ls, cuda):
     assert 0 <= shape < 4 and ms > 0
     b.advance(G2, U, V, 5, graph=True)
     cuda.cuda.synchronize()
-    assert cuda.equal(G, G2)  # per-cell arithmetic does not depend on the tiling
+    assert cuda.equal(G, G2)  # the arithmetic does not depend on the chunk length
 
 
 def test_batch_fast_shapes_match_serial(ls, cuda):

@@ -44,6 +44,8 @@ CASES = [
     (3, (21, 17, 13), (0, 0, 0)),  # reference 3D default: periodic
     (3, (67, 9, 7), (1, 1, 1)),
     (3, (19, 11, 15), (0, 2, 1)),
+    (2, (130, 5500, 1), (0, 1, 0)),  # tall enough for the 2D y-march: partial strip, chunk edges
+    (2, (130, 5500, 1), (1, 2, 0)),  # y-march with a separate ghost fill (linear extrapolation)
 ]
 
 
@@ -106,7 +108,7 @@ def test_hcr2_bitwise(ls, oracle, cuda, dim, n, bc, iters):
     assert np.array_equal(got, ref), "rel Linf %.3e" % rel_linf(got, ref)
 
 
-@pytest.mark.parametrize("dim,n,bc", [CASES[0], CASES[1], CASES[2], CASES[5], CASES[6]])
+@pytest.mark.parametrize("dim,n,bc", [CASES[0], CASES[1], CASES[2], CASES[5], CASES[6], CASES[8]])
 def test_advance_loop_bitwise(ls, oracle, cuda, dim, n, bc):
     """Time loop: sliver last step, HCR-2 after steps ri+1, 2ri+1, ... ."""
     g, p = _grid_and_params(ls, dim, n, bc, 0.1, ri=3)
