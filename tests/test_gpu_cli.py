@@ -61,6 +61,11 @@ def test_cli_sphere_reinit_matches_oracle(cuda, tmp_path, mode, tol):
     I = g.interior
     # device sin/cos may differ from glibc in the last bit: 1e-12, not bitwise
     assert rel_linf(got[I], ref[I]) <= tol
+    # the 3D driver also writes G_final.vtk (Local_code/level-set_MPI_3D/src/main.cpp:377-380)
+    # byte for byte what the writer gives for the final field
+    b = tmp_path / "o" / "G_final_oracle.vtk"
+    o.write_vtk(g, got, b)
+    assert (tmp_path / "o" / "G_final.vtk").read_bytes() == b.read_bytes()
 
 
 SAMPLE_EXE = os.path.join(ROOT, "level-set-method-code_amd", "ls_sample_solve")
