@@ -146,18 +146,16 @@ __device__ __forceinline__ int64_t off7(const AGeom& g, int q) {
     }
 }
 
-// Stage partials of one cell.  lam = cdt * lambda_out[c] (cdt = the stage's
-// coefficient of dt*RHS times dt); out[q] = lam * dRHS_c/dG[c + o_q] with
-// q = 0 the centre, 1..6 the x offsets -3,-2,-1,+1,+2,+3, 7..12 y, 13..18 z.
-// Adds the pointwise velocity and flame-speed adjoints.
-__device__ __forceinline__ void stage_partials(const AGeom& g, const double* __restrict__ Gs,
-                                               const double* __restrict__ u, const double* __restrict__ v,
-                                               const double* __restrict__ w, const double* __restrict__ sl,
-                                               double sl0, double lam, int64_t c, double (&out)[19],
-                                               double* __restrict__ gu, double* __restrict__ gv,
-                                               double* __restrict__ gw, double* __restrict__ gsl) {
+// Stage partials of one cell from its pointwise inputs (uc, vc, wc, s_l) and
+// lam = cdt * lambda_out[c] (cdt = the stage's coefficient of dt*RHS times
+// dt): out[q] = lam * dRHS_c/dG[c + o_q] with q = 0 the centre, 1..6 the x
+// offsets -3,-2,-1,+1,+2,+3, 7..12 y, 13..18 z.  Also returns the derivative
+// (dgx, dgy, dgz) and, when S_L is active, the S_L partial qim = Q/m (the
+// pointwise adjoints are -lam dg and lam qim).
+__device__ __forceinline__ bool stage_partials(const AGeom& g, const double* __restrict__ Gs, int64_t c, double uc,
+                                               double vc, double wc, double s_l, double lam, double (&out)[19],
+                                               double& dgx, double& dgy, double& dgz, double& qim) {
     const double* G = Gs + c;
-    const double s_l = sl ? sl[c] : sl0;
     const double ihx = 1.0 / g.dx, ihy = 1.0 / g.dy, ihz = 1.0 / g.dz;
     double gx = 0.0, gy = 0.0, gz = 0.0, m = 1.0, im = 1.0;
     const bool on = s_l > g.eps;
@@ -168,14 +166,14 @@ __device__ __forceinline__ void stage_partials(const AGeom& g, const double* __r
         m = sqrt(gx * gx + gy * gy + gz * gz + g.eps);
         im = lsf::rcp_nr(m);
     }
-    const double ue = u[c] - s_l * gx / m, ve = v[c] - s_l * gy / m, we = w[c] - s_l * gz / m;
+    const double ue = uc - s_l * gx / m, ve = vc - s_l * gy / m, we = wc - s_l * gz / m;
     double s7[7], Px[7], Py[7], Pz[7];
     for (int q = 0; q < 7; q++) s7[q] = G[q - 3];
-    const double dgx = dweno_grad(s7, ue, ihx, Px);
+    dgx = dweno_grad(s7, ue, ihx, Px);
     for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sy];
-    const double dgy = dweno_grad(s7, ve, ihy, Py);
+    dgy = dweno_grad(s7, ve, ihy, Py);
     for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sz];
-    const double dgz = dweno_grad(s7, we, ihz, Pz);
+    dgz = dweno_grad(s7, we, ihz, Pz);
     out[0] = -(ue * Px[3] + ve * Py[3] + we * Pz[3]);
     for (int q = 0; q < 6; q++) {
         const int t = q < 3 ? q : q + 1;
@@ -183,6 +181,7 @@ __device__ __forceinline__ void stage_partials(const AGeom& g, const double* __r
         out[7 + q] = -ve * Py[t];
         out[13 + q] = -we * Pz[t];
     }
+    qim = 0.0;
     if (on) {
         // d RHS / d g_b = s_l (dg_b - g_b Q / m^2) / m with Q = sum_a dg_a g_a
         const double Q = dgx * gx + dgy * gy + dgz * gz;
@@ -196,12 +195,10 @@ __device__ __forceinline__ void stage_partials(const AGeom& g, const double* __r
         out[10] += ry * (0.5 * ihy);
         out[15] -= rz * (0.5 * ihz);
         out[16] += rz * (0.5 * ihz);
-        if (gsl) gsl[c] += lam * (Q * im);
+        qim = Q * im;
     }
     for (int q = 0; q < 19; q++) out[q] *= lam;
-    gu[c] -= lam * dgx;
-    gv[c] -= lam * dgy;
-    gw[c] -= lam * dgz;
+    return on;
 }
 
 // Blocks of the z-marching stage adjoint: x segments of at most 64 cells
@@ -251,8 +248,16 @@ __global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8)
     for (int k = kb; k < ke; k++) {
         if (act) {
             const int64_t c = cell(g, i, j, k);
-            double out[19];
-            stage_partials(g, Gs, u, v, w, sl, sl0, cdt * lam_out[c], c, out, gu, gv, gw, gsl);
+            // every load of the cell up front: one round trip per plane
+            const double lam = cdt * lam_out[c];
+            const double uc = u[c], vc = v[c], wc = w[c], s_l = sl ? sl[c] : sl0;
+            const double gu0 = gu[c], gv0 = gv[c], gw0 = gw[c], gs0 = gsl ? gsl[c] : 0.0;
+            double out[19], dgx, dgy, dgz, qim;
+            const bool on = stage_partials(g, Gs, c, uc, vc, wc, s_l, lam, out, dgx, dgy, dgz, qim);
+            gu[c] = gu0 - lam * dgx;
+            gv[c] = gv0 - lam * dgy;
+            gw[c] = gw0 - lam * dgz;
+            if (gsl && on) gsl[c] = gs0 + lam * qim;
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 sX[ly][q][lx] = out[1 + q];
