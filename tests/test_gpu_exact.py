@@ -66,7 +66,7 @@ def test_rk3_step_bitwise(ls, oracle, cuda, dim, n, bc, s_l):
     assert np.array_equal(got, ref), "rel Linf %.3e" % rel_linf(got, ref)
 
 
-@pytest.mark.parametrize("dim,n,bc", [CASES[0], CASES[1], CASES[5]])
+@pytest.mark.parametrize("dim,n,bc", [CASES[0], CASES[1], CASES[5], CASES[8]])
 def test_rk3_step_sl_field_bitwise(ls, oracle, cuda, dim, n, bc):
     g, p = _grid_and_params(ls, dim, n, bc, 0.0)
     G, vel = _inputs(g, seed=11)

@@ -69,6 +69,23 @@ def test_fast_step_and_hcr2(ls, oracle, cuda, dim, n, bc, s_l):
     front_parity(dH.cpu().numpy(), refH, h, TOL, I)
 
 
+@pytest.mark.parametrize("n", [(23, 19, 1), (130, 5500, 1)])  # one thread per cell / y-march
+def test_fast_sl_field_2d(ls, oracle, cuda, n):
+    """A per-cell S_L field (values below the 1e-10 gate included)."""
+    bc = (0, 1, 0)
+    g = make_grid(2, *n, bc=bc, s_l=0.0)
+    p = ls.default_params(2, n[0], nx=n[0], ny=n[1], bc=bc, s_l=0.0, mode=ls.MODE_FAST)
+    G = oracle.fill_ghosts(g, bumpy_sdf(g.shape, _h(p), 23))
+    vel = rand_vel(g.shape, 23, 1.0, n=2)
+    sl = np.random.default_rng(6).uniform(-0.05, 0.3, g.shape)
+    s = ls.Solver(p)
+    out = s.empty()
+    dt = 0.3 * min(g.dx, g.dy)
+    s.rk3_step(_dev(cuda, G), out, _dev(cuda, vel[0]), _dev(cuda, vel[1]), None, dt=dt, sl=_dev(cuda, sl))
+    ref = oracle.rk3_step(g, G, vel[0], vel[1], None, dt=dt, sl=sl)
+    assert rel_linf(out.cpu().numpy(), ref, g.interior) <= TOL
+
+
 def test_fast_config1_full(ls, oracle, cuda):
     p = ls.default_params(2, 256, s_l=0.1, reinit_interval=10, bc=(1, 1, 0), mode=ls.MODE_FAST)
     G, u, v, _ = ls.init_case(p, ls.CASE_CIRCLE2D)
