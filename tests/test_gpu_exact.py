@@ -66,6 +66,36 @@ def test_rk3_step_bitwise(ls, oracle, cuda, dim, n, bc, s_l):
     assert np.array_equal(got, ref), "rel Linf %.3e" % rel_linf(got, ref)
 
 
+@pytest.mark.parametrize("dim,n,bc", [CASES[0], CASES[5], CASES[8]])
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_rk3_stage_distinct_fields_bitwise(ls, oracle, cuda, dim, n, bc, k):
+    """ls_rk3_stage with G_s and G_n distinct (also for stage 1): interior =
+    the reference's stage form (rk3.h:103-150) of G_n, G_s and RHS(G_s), then
+    the ghost fill."""
+    g, p = _grid_and_params(ls, dim, n, bc, 0.2)
+    Gs, vel = _inputs(g, seed=13)
+    Gs = oracle.fill_ghosts(g, Gs)
+    Gn = oracle.fill_ghosts(g, _inputs(g, seed=14)[0])
+    w = vel[2] if dim == 3 else None
+    dt = 1.5e-3
+    s = ls.Solver(p)
+    out = s.empty()
+    s.rk3_stage(k, _dev(cuda, Gs), _dev(cuda, Gn), out, _dev(cuda, vel[0]), _dev(cuda, vel[1]), _dev(cuda, w),
+                dt=dt)
+    r = oracle.rhs(g, Gs, vel[0], vel[1], w)
+    if k == 1:
+        e = Gn + dt * r
+    elif k == 2:
+        e = 0.75 * Gn + 0.25 * Gs + 0.25 * dt * r
+    else:
+        e = (1.0 / 3.0) * Gn + (2.0 / 3.0) * Gs + (2.0 / 3.0) * dt * r
+    I = g.interior
+    ref = Gn.copy()
+    ref[I] = e[I]
+    ref = oracle.fill_ghosts(g, ref)
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
 @pytest.mark.parametrize("dim,n,bc", [CASES[0], CASES[1], CASES[5], CASES[8]])
 def test_rk3_step_sl_field_bitwise(ls, oracle, cuda, dim, n, bc):
     g, p = _grid_and_params(ls, dim, n, bc, 0.0)
