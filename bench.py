@@ -1,6 +1,6 @@
 """Benchmark: 512^3 FP64 G-equation WENO5-RK3 + HCR-2 on one MI355X.
 
-Workload (the 3D roofline configuration): 512^3 interior cells on the unit
+Workload (the 3D roofline workload): 512^3 interior cells on the unit
 cube, periodic, G0 = x - 0.3 - eta(y,z) with 8 seeded surface modes, velocity
 (1,0,0) + 16 seeded solenoidal modes of amplitude 0.1, S_L = 0.08, CFL 0.5,
 HCR-2 (2 iterations) after every 5th RK3 step.  A step is one RK3 step plus
@@ -21,7 +21,7 @@ Prints ONE JSON line (rank 0).  Options:
                            only (for rocprofv3 --pmc runs)
   --summarize-pmc F W      turn two rocprofv3 --pmc CSV directories (FETCH_SIZE,
                            WRITE_SIZE) into profiles/stage_kernel_traffic_512_<mode>.json
-  --config batch           the batched-ensemble configuration instead: 96 2D
+  --config batch           the batched-ensemble workload instead: 96 2D
                            flames of seeded sizes in {128,192,256,384,512}^2,
                            S_L ~ U[0.05,0.2], 300 RK3 steps each, in one
                            autotuned, hipGraph-replayed batch vs a serial
@@ -29,7 +29,7 @@ Prints ONE JSON line (rank 0).  Options:
   --config sample          the reference's sample/ scheme instead: its 800x800
                            Bunsen-flame driver loop, 2000 steps, next to the
                            reference's own sample code on 16 CPU threads
-  --config adjoint         the adjoint configuration instead: flame-inversion
+  --config adjoint         the adjoint workload instead: flame-inversion
                            gradients (128^3, 8^3 S_L nodes + 16 mode amplitudes,
                            100 steps) for --sets seeded parameter sets vs
                            central finite differences on the forward solver
@@ -144,7 +144,7 @@ def cpu_baseline(n):
 
 
 def ensemble(ls, count=96, seed=4, mode="fast"):
-    """The batched configuration: seeded member sizes, flame speeds and inputs."""
+    """The batched workload: seeded member sizes, flame speeds and inputs."""
     import numpy as np
     import torch
     rng = np.random.default_rng(seed)
@@ -222,7 +222,7 @@ def bench_batch(ls, steps, mode, emit=True):
 
 
 def bench_adjoint(ls, mode, nsets, fd_sample=16, emit=True):
-    """The adjoint configuration: parameter gradients of the flame-inversion
+    """The adjoint workload: parameter gradients of the flame-inversion
     loss for `nsets` seeded parameter sets vs central finite differences on
     the plain forward solver (2 forward solves per parameter per gradient)."""
     import numpy as np
@@ -470,7 +470,7 @@ def main():
     ap.add_argument("--profile-pass", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--traffic-out", default=None)
-    ap.add_argument("--config", choices=["c3", "batch", "adjoint", "sample"], default="c3")
+    ap.add_argument("--config", choices=["plane_modes", "batch", "adjoint", "sample"], default="plane_modes")
     ap.add_argument("--sets", type=int, default=32)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
@@ -482,8 +482,8 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus > 1 or world > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the solver would "
-              "shard by y-slabs with 3-layer halo exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the solver has no multi-GPU path (it would shard by "
+              "y-slabs with a 3-layer halo exchange per stage)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -3,7 +3,7 @@
 // Shapes and velocity fields of the reference test cases
 // (Local_code/level-set_MPI_2D/include/initial_conditions.h:67-117,209-281;
 //  Local_code/level-set_MPI_3D/include/initial_conditions.h:52-58,232-343)
-// and the seeded workloads of the benchmark configurations.  Seeded draws use
+// and the seeded workloads of the benchmark.  Seeded draws use
 // SplitMix64 on the host; the kernels receive the drawn mode tables by value.
 #include <hip/hip_runtime.h>
 

@@ -37,7 +37,7 @@ def _dev(cuda, a):
 
 CASES = [
     (2, (23, 19, 1), (0, 1, 0)),   # reference 2D default: periodic x, zero-gradient y
-    (2, (23, 19, 1), (1, 1, 0)),   # zero-gradient all sides (config 1)
+    (2, (23, 19, 1), (1, 1, 0)),   # zero-gradient all sides (the 2D circle workload)
     (2, (23, 19, 1), (0, 2, 0)),   # linear extrapolation in y
     (2, (17, 29, 1), (1, 2, 0)),
     (2, (70, 9, 1), (0, 0, 0)),    # fully periodic (no reference counterpart in 2D)

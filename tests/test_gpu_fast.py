@@ -86,7 +86,7 @@ def test_fast_sl_field_2d(ls, oracle, cuda, n):
     assert rel_linf(out.cpu().numpy(), ref, g.interior) <= TOL
 
 
-def test_fast_config1_full(ls, oracle, cuda):
+def test_fast_circle2d_256_full(ls, oracle, cuda):
     p = ls.default_params(2, 256, s_l=0.1, reinit_interval=10, bc=(1, 1, 0), mode=ls.MODE_FAST)
     G, u, v, _ = ls.init_case(p, ls.CASE_CIRCLE2D)
     s = ls.Solver(p)
@@ -101,11 +101,11 @@ def test_fast_config1_full(ls, oracle, cuda):
     front_parity(got, ref, _h(p), TOL, g.interior)
 
 
-@pytest.mark.parametrize("case,n,steps", [("c2", 64, 100), ("c3", 48, 40)])
-def test_fast_configs_reduced(ls, oracle, cuda, case, n, steps):
-    s_l = 0.05 if case == "c2" else 0.08
+@pytest.mark.parametrize("case,n,steps", [("sphere_swirl", 64, 100), ("plane_modes", 48, 40)])
+def test_fast_3d_workloads_reduced(ls, oracle, cuda, case, n, steps):
+    s_l = 0.05 if case == "sphere_swirl" else 0.08
     p = ls.default_params(3, n, s_l=s_l, reinit_interval=5, mode=ls.MODE_FAST)
-    cid = ls.CASE_SPHERE_SWIRL3D if case == "c2" else ls.CASE_PLANE_MODES3D
+    cid = ls.CASE_SPHERE_SWIRL3D if case == "sphere_swirl" else ls.CASE_PLANE_MODES3D
     G, u, v, w = ls.init_case(p, cid, seed=3)
     s = ls.Solver(p)
     s.fill_ghosts(G)
@@ -119,16 +119,17 @@ def test_fast_configs_reduced(ls, oracle, cuda, case, n, steps):
     front_parity(got, ref, _h(p), TOL, g.interior)
 
 
-@pytest.mark.parametrize("case,n,steps", [("c2", 256, 40), ("c3", 512, 1), ("c3", 512, 100)])
+@pytest.mark.parametrize("case,n,steps", [("sphere_swirl", 256, 40), ("plane_modes", 512, 1), ("plane_modes", 512, 100)])
 def test_fast_full_size_vs_exact(ls, cuda, case, n, steps):
-    """Full-size configurations 2 and 3: fast mode against the exact mode, one
+    """The full-size 3D workloads (sphere in swirl 256^3, plane with modes 512^3):
+    fast mode against the exact mode, one
     step and over the benchmark's 100-step horizon (HCR-2 every 5 steps).
     The gap is not monotonic in time: discrete events near zero (an upwind
     choice at u_eff ~ 0, an HCR-2 stability test) can open a local ~1e-12
     difference that the scheme damps again; measured at 512^3 it is 4e-16
     after 1 step, ~8e-13 after 20 and ~2e-14 after 100."""
-    s_l = 0.05 if case == "c2" else 0.08
-    cid = ls.CASE_SPHERE_SWIRL3D if case == "c2" else ls.CASE_PLANE_MODES3D
+    s_l = 0.05 if case == "sphere_swirl" else 0.08
+    cid = ls.CASE_SPHERE_SWIRL3D if case == "sphere_swirl" else ls.CASE_PLANE_MODES3D
     res = []
     for mode in (ls.MODE_EXACT, ls.MODE_FAST):
         p = ls.default_params(3, n, s_l=s_l, reinit_interval=5, mode=mode)

@@ -11,11 +11,12 @@
    the logged runs were compiled with (-O3 -march=native): about 120 cells of
    the Zalesak disk sit within 1e-15 of zero, and only the contracted
    arithmetic reproduces the logged initial areas.
-2. Configuration 1 (2D 256^2 circle, 200 steps, HCR-2 every 10) at full size
-   and configurations 2 and 3 at reduced size: bitwise against the oracle.
-3. Configurations 2 (256^3) and 3 (512^3) at full size, where the CPU oracle
-   would take hours: periodic translation equivariance (bitwise) and
-   run-to-run determinism.
+2. The 2D 256^2 circle workload (200 steps, HCR-2 every 10) at full size, and
+   the 3D sphere-in-swirl and plane-with-modes workloads at reduced size:
+   bitwise against the oracle.
+3. The sphere-in-swirl (256^3) and plane-with-modes (512^3) workloads at full
+   size, where the CPU oracle would take hours: periodic translation
+   equivariance (bitwise) and run-to-run determinism.
 """
 import math
 
@@ -80,7 +81,7 @@ def test_sphere_deformation_64_log_known_answers(ls, cuda):
     assert "%.6f" % vol0 == "0.014165" and "%.6f" % vol1 == "0.014245"
 
 
-def test_config1_full_bitwise(ls, oracle, cuda):
+def test_circle2d_256_full_bitwise(ls, oracle, cuda):
     """256^2 circle r=0.25, u=(0.5,0), S_L=0.1, CFL 0.5, 200 steps, HCR-2 every 10,
     zero-gradient on all sides."""
     p = ls.default_params(2, 256, s_l=0.1, reinit_interval=10, bc=(1, 1, 0))
@@ -98,7 +99,7 @@ def test_config1_full_bitwise(ls, oracle, cuda):
 
 
 @pytest.mark.parametrize("n,steps", [(48, 60), (64, 25)])
-def test_config2_reduced_bitwise(ls, oracle, cuda, n, steps):
+def test_sphere_swirl3d_reduced_bitwise(ls, oracle, cuda, n, steps):
     """Sphere r=0.2 in the swirl field, S_L=0.05, CFL 0.5, HCR-2 every 5, periodic."""
     p = ls.default_params(3, n, s_l=0.05, reinit_interval=5)
     G, u, v, w = ls.init_case(p, ls.CASE_SPHERE_SWIRL3D)
@@ -112,7 +113,7 @@ def test_config2_reduced_bitwise(ls, oracle, cuda, n, steps):
     assert np.array_equal(G.cpu().numpy(), ref)
 
 
-def test_config3_reduced_bitwise(ls, oracle, cuda):
+def test_plane_modes3d_reduced_bitwise(ls, oracle, cuda):
     """Plane x=0.3 with 8 seeded modes, u=(1,0,0)+16 solenoidal modes, S_L=0.08."""
     n, steps = 40, 20
     p = ls.default_params(3, n, s_l=0.08, reinit_interval=5)
@@ -141,13 +142,13 @@ def _roll_interior(cuda, a, shift, ng=3):
     return out
 
 
-@pytest.mark.parametrize("case,n,steps,s_l", [("c2", 256, 6, 0.05), ("c3", 512, 6, 0.08)])
+@pytest.mark.parametrize("case,n,steps,s_l", [("sphere_swirl", 256, 6, 0.05), ("plane_modes", 512, 6, 0.08)])
 def test_full_size_translation_equivariance(ls, cuda, case, n, steps, s_l):
     """With periodic boundaries every cell sees the same stencil values after a
     whole-cell periodic shift, so shifting the inputs must shift the output
     bit for bit (RK3 stages, ghost fills and HCR-2 included)."""
     p = ls.default_params(3, n, s_l=s_l, reinit_interval=5)
-    cid = ls.CASE_SPHERE_SWIRL3D if case == "c2" else ls.CASE_PLANE_MODES3D
+    cid = ls.CASE_SPHERE_SWIRL3D if case == "sphere_swirl" else ls.CASE_PLANE_MODES3D
     G, u, v, w = ls.init_case(p, cid, seed=3)
     s = ls.Solver(p)
     s.fill_ghosts(G)
