@@ -38,6 +38,8 @@ struct BMember {
     int64_t off;
     double dx, dy, s_l, dt, eps;
     double inv2dx, inv2dy, inv6dx, inv6dy;
+    double rdx, rdy;
+    int32_t crdiv, pad2;
 };
 
 struct BTile {
@@ -86,6 +88,9 @@ __global__ __launch_bounds__(lsf::kYW) void k_batch_ymarch(const BMember* __rest
         m.inv6dy = b.inv6dy;
         m.dx = b.dx;
         m.dy = b.dy;
+        m.rdx = b.rdx;
+        m.rdy = b.rdy;
+        m.crdiv = b.crdiv;
         const int y0 = tl.ty * ch;
         lsf::ymarch_strip<STAGE, EXACT>(m, tl.tx * lsf::kYW, y0, min(y0 + ch, b.ny), Gs, Gn, out, u, v, nullptr, L);
     }
@@ -223,6 +228,9 @@ int ls_batch_create(const ls_batch_member_t* members, int32_t count, int32_t mod
         d.inv2dy = 1.0 / (2.0 * q.dy);
         d.inv6dx = 1.0 / (6.0 * q.dx);
         d.inv6dy = 1.0 / (6.0 * q.dy);
+        d.rdx = 1.0 / q.dx;
+        d.rdy = 1.0 / q.dy;
+        d.crdiv = lsx::crdiv_geom_ok(q.dx, q.dy, 1.0, d.eps) && lsx::crdiv_safe(q.s_l);
         b->hmem.push_back(d);
         b->offs.push_back(off);
         const int64_t n = (int64_t)(q.nx + 6) * (q.ny + 6);

@@ -194,6 +194,8 @@ struct ZGeom {
     double s_l, eps;
     double inv2dx, inv2dy, inv2dz, inv6dx, inv6dy, inv6dz;
     double dx, dy, dz;            // exact-mode stage kernel: the reference divides by these
+    double rdx, rdy, rdz;         // RN(1/dx), RN(1/dy), RN(1/dz) for lsx::qdiv_u
+    int32_t crdiv;                // spacings, eps and a scalar S_L admit the FMA division sequences
 };
 
 
@@ -264,16 +266,25 @@ __device__ __forceinline__ double2 as2(double f) { return make_double2(f, 1.0); 
 // weno5_right on G(c-1..c+3) for the face c+1/2 (weno5.h:39-81), in its own
 // operation order.  A face value depends only on those five values, so the
 // two cells beside a face get the same bits whether it is built once or
-// twice.  g points at G(c) with stride GS.
-template <int GS>
+// twice.  g points at G(c) with stride GS; D is the division flavour
+// (ls_crdiv.hpp).
+template <int GS, int D>
 __device__ __forceinline__ double xface_lds(const double* g, bool mirror) {
-    return mirror ? lsx::weno_r(g[-GS], g[0], g[GS], g[2 * GS], g[3 * GS])
-                  : lsx::weno_l(g[-2 * GS], g[-GS], g[0], g[GS], g[2 * GS]);
+    return mirror ? lsx::weno_r<D>(g[-GS], g[0], g[GS], g[2 * GS], g[3 * GS])
+                  : lsx::weno_l<D>(g[-2 * GS], g[-GS], g[0], g[GS], g[2 * GS]);
 }
+
+using DivIEEE = std::integral_constant<int, lsx::kDivIEEE>;
+using DivFma = std::integral_constant<int, lsx::kDivFma>;
 
 // EXACT: the exact mode's arithmetic (bitwise the CPU oracle) on the same
 // shared-face z-march; faces are then single values and the smoothness terms
-// are computed inside each reconstruction, as the reference does.
+// are computed inside each reconstruction, as the reference does.  The
+// divisions run as the FMA sequences of ls_crdiv.hpp while every G and S_L
+// value the block has loaded passed crdiv_safe (each thread checks what it
+// loads; a sticky block flag in LDS collects the failures) and the host found
+// the spacings in range (z.crdiv); otherwise as divisions.  Both give the
+// same bits, so waves may differ in which they use.
 template <int STAGE, bool EXACT>
 __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_stage_zface(
     ZGeom z, const double* __restrict__ Gs, const double* __restrict__ Gn, double* __restrict__ out,
@@ -286,6 +297,7 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
     __shared__ FV sFx[2][FY][FX + 1];                    // [left-biased, mirrored] x face between local x s-1 and s
     __shared__ FV sFy[2][FY + 1][FX];                    // [left-biased, mirrored] y face between local y s-1 and s
     __shared__ unsigned char sOri[FY][FX];     // bit 0: u_eff >= 0, bit 1: v_eff >= 0
+    __shared__ int sBad;                       // exact mode: some loaded value failed crdiv_safe
     const int64_t t = xcd_tile(blockIdx.x, z.nblocks);
     if (t >= z.ntiles) return;  // block-uniform
     const int tx = (int)(t % z.tiles_x);
@@ -301,6 +313,11 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
     const int64_t sz = z.sz;
     const bool same = Gs == Gn;
     const double c2 = 0.25 * dt, c3 = (2.0 / 3.0) * dt;
+    bool bad = false;  // exact mode: a value this thread loaded failed crdiv_safe
+    if (EXACT) {
+        if (tid == 0) sBad = 0;
+        __syncthreads();  // before any thread can raise the flag
+    }
 
     // Loads are unconditional: a position past the padded array reads the
     // nearest padded cell instead.  Such positions feed only the stencils of
@@ -346,7 +363,10 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
         sG[ly + 3][lx + 3] = own;
 #pragma unroll
         for (int a = 0; a < NH; a++)
-            if (tid + a * kFaceThreads < kHaloItems) sG[hrow[a]][hcol[a]] = n.halo[a];
+            if (tid + a * kFaceThreads < kHaloItems) {
+                sG[hrow[a]][hcol[a]] = n.halo[a];
+                if (EXACT) bad |= !lsx::crdiv_safe(n.halo[a]);
+            }
     };
 
     // Column window along z: zG[q] = G(k-3+q), zt[q] = T(k-2+q).
@@ -354,6 +374,9 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
 #pragma unroll
     for (int q = 0; q < 6; q++) zG[q] = Gs[(int64_t)(kb + q) * sz + col];
     zG[6] = 0.0;
+    if (EXACT)
+#pragma unroll
+        for (int q = 0; q < 6; q++) bad |= !lsx::crdiv_safe(zG[q]);
 #pragma unroll
     for (int q = 0; q < 4; q++) zt[q] = tsq(zG[q], zG[q + 1], zG[q + 2]);
     zt[4] = 0.0;
@@ -369,8 +392,14 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
         zG[6] = nx.lead;
         zt[4] = tsq(zG[4], zG[5], zG[6]);
         const double cu = nx.u, cv = nx.v, cw = nx.w, cgn = same ? zG[3] : nx.gn, csl = nx.sl;
+        if (EXACT) {
+            bad |= !lsx::crdiv_safe(zG[6]) || (sl && !lsx::crdiv_safe(csl));
+            if (bad) sBad = 1;  // sticky; read after the barrier
+        }
         if (k + 1 < ke) fetch(k + 1, nx);
         __syncthreads();  // plane k in sG
+        // every value of plane k's stencils was checked before this barrier
+        const bool fd = EXACT && z.crdiv && __builtin_amdgcn_readfirstlane(sBad) == 0;
         // --- smoothness terms along x and y, effective velocity, orientation
         if (!EXACT) {
         sTx[ly][lx + 2] = tsq(sG[ly + 3][lx + 2], sG[ly + 3][lx + 3], sG[ly + 3][lx + 4]);
@@ -387,16 +416,22 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
         }
         double ue = cu, ve = cv, we = cw;
         if (EXACT) {  // computeRHS3D (rk3.h:31-81), C++ evaluation order
-            double gx = 0.0, gy = 0.0, gz = 0.0, gm = 1.0;
-            if (csl > z.eps) {
-                gx = (sG[ly + 3][lx + 4] - sG[ly + 3][lx + 2]) / (2.0 * z.dx);
-                gy = (sG[ly + 4][lx + 3] - sG[ly + 2][lx + 3]) / (2.0 * z.dy);
-                gz = (zG[4] - zG[2]) / (2.0 * z.dz);
-                gm = sqrt(gx * gx + gy * gy + gz * gz + z.eps);
-            }
-            ue = cu - csl * gx / gm;
-            ve = cv - csl * gy / gm;
-            we = cw - csl * gz / gm;
+            auto ueff = [&](auto dtag) {
+                constexpr int D = decltype(dtag)::value;
+                double gx = 0.0, gy = 0.0, gz = 0.0, gm = 1.0;
+                if (csl > z.eps) {  // 2h is exact, and RN(1/(2h)) = RN(1/h) / 2
+                    gx = lsx::dq<D>(sG[ly + 3][lx + 4] - sG[ly + 3][lx + 2], 2.0 * z.dx, 0.5 * z.rdx);
+                    gy = lsx::dq<D>(sG[ly + 4][lx + 3] - sG[ly + 2][lx + 3], 2.0 * z.dy, 0.5 * z.rdy);
+                    gz = lsx::dq<D>(zG[4] - zG[2], 2.0 * z.dz, 0.5 * z.rdz);
+                    gm = sqrt(gx * gx + gy * gy + gz * gz + z.eps);
+                }
+                const double ygm = D == lsx::kDivIEEE ? 0.0 : 1.0 / gm;
+                ue = cu - lsx::dq<D>(csl * gx, gm, ygm);
+                ve = cv - lsx::dq<D>(csl * gy, gm, ygm);
+                we = cw - lsx::dq<D>(csl * gz, gm, ygm);
+            };
+            if (fd) ueff(DivFma());
+            else ueff(DivIEEE());
         } else if (csl > z.eps) {
             const double gx = (sG[ly + 3][lx + 4] - sG[ly + 3][lx + 2]) * z.inv2dx;
             const double gy = (sG[ly + 4][lx + 3] - sG[ly + 2][lx + 3]) * z.inv2dy;
@@ -416,8 +451,13 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
             constexpr int GS = decltype(gs)::value, TS = decltype(ts)::value, OS = decltype(os)::value;
             if constexpr (EXACT) {
                 (void)t;
-                if (nL) slot[0] = xface_lds<GS>(g, false);
-                if (nR) slot[OS] = xface_lds<GS>(g, true);
+                if (fd) {
+                    if (nL) slot[0] = xface_lds<GS, lsx::kDivFma>(g, false);
+                    if (nR) slot[OS] = xface_lds<GS, lsx::kDivFma>(g, true);
+                } else {
+                    if (nL) slot[0] = xface_lds<GS, lsx::kDivIEEE>(g, false);
+                    if (nR) slot[OS] = xface_lds<GS, lsx::kDivIEEE>(g, true);
+                }
             } else {
                 face_put<GS, TS, OS>(g, t, nL, nR, slot);
             }
@@ -453,11 +493,16 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
             }
             if constexpr (EXACT) {
                 // z faces as single values in zlo.n
-                if (zlo_L != (int)Lz)
-                    zlo.n = Lz ? lsx::weno_l(zG[0], zG[1], zG[2], zG[3], zG[4])
-                               : lsx::weno_r(zG[1], zG[2], zG[3], zG[4], zG[5]);
-                zhi.n = Lz ? lsx::weno_l(zG[1], zG[2], zG[3], zG[4], zG[5])
-                           : lsx::weno_r(zG[2], zG[3], zG[4], zG[5], zG[6]);
+                auto zfaces = [&](auto dtag) {
+                    constexpr int D = decltype(dtag)::value;
+                    if (zlo_L != (int)Lz)
+                        zlo.n = Lz ? lsx::weno_l<D>(zG[0], zG[1], zG[2], zG[3], zG[4])
+                                   : lsx::weno_r<D>(zG[1], zG[2], zG[3], zG[4], zG[5]);
+                    zhi.n = Lz ? lsx::weno_l<D>(zG[1], zG[2], zG[3], zG[4], zG[5])
+                               : lsx::weno_r<D>(zG[2], zG[3], zG[4], zG[5], zG[6]);
+                };
+                if (fd) zfaces(DivFma());
+                else zfaces(DivIEEE());
             } else {
                 // z face k-1/2 in this cell's orientation (carried unless it differs)
                 if (zlo_L != (int)Lz) {
@@ -488,9 +533,19 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
             // dweno_*_3d: (face(c+1/2) - face(c-1/2)) / h; rhs and stage in the
             // reference's order (rk3.h:31-125)
             const int ox = Lx ? 0 : 1, oy = Ly ? 0 : 1;
-            const double dgx = (val(sFx[ox][ly][lx + 1]) - val(sFx[ox][ly][lx])) / z.dx;
-            const double dgy = (val(sFy[oy][ly + 1][lx]) - val(sFy[oy][ly][lx])) / z.dy;
-            const double dgz = (zhi.n - zlo.n) / z.dz;
+            const double ax = val(sFx[ox][ly][lx + 1]) - val(sFx[ox][ly][lx]);
+            const double ay = val(sFy[oy][ly + 1][lx]) - val(sFy[oy][ly][lx]);
+            const double az = zhi.n - zlo.n;
+            double dgx, dgy, dgz;
+            if (fd) {
+                dgx = lsx::qdiv_u(ax, z.dx, z.rdx);
+                dgy = lsx::qdiv_u(ay, z.dy, z.rdy);
+                dgz = lsx::qdiv_u(az, z.dz, z.rdz);
+            } else {
+                dgx = ax / z.dx;
+                dgy = ay / z.dy;
+                dgz = az / z.dz;
+            }
             const double rr = -(ue * dgx + ve * dgy + we * dgz);
             const double o = lsx::stage_combine<STAGE>(cgn, zG[3], rr, dt);
             out[plane + col] = o;
@@ -841,6 +896,10 @@ static ZGeom make_z(const Geom& g, int kz, int tile_cols = ZX, int tile_rows = Z
     z.dx = g.dx;
     z.dy = g.dy;
     z.dz = g.dz;
+    z.rdx = g.rdx;
+    z.rdy = g.rdy;
+    z.rdz = g.rdz;
+    z.crdiv = lsx::crdiv_geom_ok(g.dx, g.dy, g.dz, g.eps) && lsx::crdiv_safe(g.s_l);
     return z;
 }
 
@@ -909,6 +968,9 @@ static YGrid make_y(const Geom& g, double dt) {
     m.inv6dy = f.inv6dy;
     m.dx = g.dx;
     m.dy = g.dy;
+    m.rdx = g.rdx;
+    m.rdy = g.rdy;
+    m.crdiv = lsx::crdiv_geom_ok(g.dx, g.dy, 1.0, g.eps) && lsx::crdiv_safe(g.s_l);
     return m;
 }
 

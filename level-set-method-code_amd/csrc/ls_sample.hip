@@ -30,6 +30,7 @@
 #include <cstring>
 
 #include "ls_api.h"
+#include "ls_crdiv.hpp"
 #include "ls_internal.hpp"
 
 namespace {
@@ -40,6 +41,9 @@ constexpr double kWenoEps = 1e-6;
 struct SGeom {
     int32_t nx, ny, nxt, nyt;
     double dx, dy;
+    double rdx, rdy;  // RN(1/dx), RN(1/dy) for lsx::qdiv
+    int32_t crdiv;    // the spacings admit the FMA division sequences (ls_crdiv.hpp)
+    int32_t pad;
 };
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
@@ -51,7 +55,9 @@ __device__ __forceinline__ double sbeta(double f0, double f1, double f2) {
     return (13.0 / 12.0) * (p * p) + (1.0 / 4.0) * (q * q);
 }
 
-// weno5_left of the reference (sample/weno5_rk3.cpp:17-32).
+// weno5_left of the reference (sample/weno5_rk3.cpp:17-32); D is the division
+// flavour (ls_crdiv.hpp), every choice giving the same bits.
+template <int D = lsx::kDivChecked>
 __device__ __forceinline__ double sface(double v0, double v1, double v2, double v3, double v4) {
     const double b0 = sbeta(v0, v1, v2);
     const double b1 = sbeta(v1, v2, v3);
@@ -60,29 +66,32 @@ __device__ __forceinline__ double sface(double v0, double v1, double v2, double 
     const double a1 = 0.6 / ((kWenoEps + b1) * (kWenoEps + b1));
     const double a2 = 0.3 / ((kWenoEps + b2) * (kWenoEps + b2));
     const double sum = a0 + a1 + a2;
-    const double w0 = a0 / sum, w1 = a1 / sum, w2 = a2 / sum;
-    return w0 * (2.0 * v0 - 7.0 * v1 + 11.0 * v2) / 6.0 + w1 * (-v1 + 5.0 * v2 + 2.0 * v3) / 6.0 +
-           w2 * (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
+    const double ys = D == lsx::kDivIEEE ? 0.0 : 1.0 / sum;
+    const double w0 = lsx::dq<D>(a0, sum, ys), w1 = lsx::dq<D>(a1, sum, ys), w2 = lsx::dq<D>(a2, sum, ys);
+    return lsx::d6<D>(w0 * (2.0 * v0 - 7.0 * v1 + 11.0 * v2)) + lsx::d6<D>(w1 * (-v1 + 5.0 * v2 + 2.0 * v3)) +
+           lsx::d6<D>(w2 * (2.0 * v2 + 5.0 * v3 - v4));
 }
 
 // compute_geqn_rhs at one interior cell (sample/weno5_rk3.cpp:111-143) from
 // its three faces per axis, L at i-1/2, i+1/2, i+3/2 (dG_L and dG_R of
 // weno5_rk3.cpp:53-87 are their differences over h).
+template <int D = lsx::kDivChecked>
 __device__ __forceinline__ double srhs_f(const SGeom& g, double fmx, double f0x, double fpx, double fmy, double f0y,
                                          double fpy, double uc, double vc, double S_L) {
-    const double Gdxm = (f0x - fmx) / g.dx, Gdxp = (fpx - f0x) / g.dx;
-    const double Gdym = (f0y - fmy) / g.dy, Gdyp = (fpy - f0y) / g.dy;
+    const double Gdxm = lsx::dq<D>(f0x - fmx, g.dx, g.rdx), Gdxp = lsx::dq<D>(fpx - f0x, g.dx, g.rdx);
+    const double Gdym = lsx::dq<D>(f0y - fmy, g.dy, g.rdy), Gdyp = lsx::dq<D>(fpy - f0y, g.dy, g.rdy);
     const double Gdx_mean = 0.5 * (Gdxm + Gdxp);
     const double Gdy_mean = 0.5 * (Gdym + Gdyp);
     const double transport = uc * Gdx_mean + vc * Gdy_mean;
     const double propagation = S_L * sqrt(Gdx_mean * Gdx_mean + Gdy_mean * Gdy_mean);
     const double nm = sqrt(Gdxm * Gdxm + Gdym * Gdym + 1e-14);
     const double np = sqrt(Gdxp * Gdxp + Gdyp * Gdyp + 1e-14);
-    const double s_xm = uc - S_L * Gdxm / nm;
-    const double s_xp = uc - S_L * Gdxp / np;
+    const double ynm = D == lsx::kDivIEEE ? 0.0 : 1.0 / nm, ynp = D == lsx::kDivIEEE ? 0.0 : 1.0 / np;
+    const double s_xm = uc - lsx::dq<D>(S_L * Gdxm, nm, ynm);
+    const double s_xp = uc - lsx::dq<D>(S_L * Gdxp, np, ynp);
     const double alpha_x = dmax(fabs(s_xm), fabs(s_xp));
-    const double s_ym = vc - S_L * Gdym / nm;
-    const double s_yp = vc - S_L * Gdyp / np;
+    const double s_ym = vc - lsx::dq<D>(S_L * Gdym, nm, ynm);
+    const double s_yp = vc - lsx::dq<D>(S_L * Gdyp, np, ynp);
     const double alpha_y = dmax(fabs(s_ym), fabs(s_yp));
     const double alpha = dmax(alpha_x, alpha_y);
     const double diss_x = alpha * (Gdxp - Gdxm) * 0.5;
@@ -127,7 +136,10 @@ __device__ __forceinline__ int s_img(int q, int n, int a) {
 // per-cell one.  Row 3 takes the initial field; row ny+2 (padded) is
 // 2 row(ny+1) - row(ny) of this stage's output, which the same wavefront has
 // just computed (the last chunk holds at least 3 rows).  out may alias Gn (stage 3):
-// a cell reads G_n only at itself, before writing it.
+// a cell reads G_n only at itself, before writing it.  The divisions run as
+// the FMA sequences of ls_crdiv.hpp while every G value the wavefront has
+// loaded passed lsx::crdiv_safe (a sticky per-lane flag and a wave vote), S_L
+// does, and g.crdiv is set; otherwise as divisions.  Both give the same bits.
 constexpr int kSW = 62;
 #ifndef LS_SAMPLE_ROWS
 #define LS_SAMPLE_ROWS 4  // rows per chunk but the last
@@ -163,8 +175,18 @@ __global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int n
     double ring[7];  // G(X, r-3 .. r+3)
 #pragma unroll
     for (int q = 0; q < 7; q++) ring[q] = col ? Gs[(int64_t)(r0 - 3 + q) * s + X] : 0.0;
-    double fy0 = sface(ring[0], ring[1], ring[2], ring[3], ring[4]);  // y face r-1/2
-    double fy1 = sface(ring[1], ring[2], ring[3], ring[4], ring[5]);  // y face r+1/2
+    bool bad = false;  // a G value this lane loaded failed crdiv_safe
+#pragma unroll
+    for (int q = 0; q < 7; q++) bad |= !lsx::crdiv_safe(ring[q]);
+    const bool fd_ok = g.crdiv && lsx::crdiv_safe(S_L);
+    double fy0, fy1;  // y faces r-1/2, r+1/2
+    if (fd_ok && __ballot(bad) == 0) {
+        fy0 = sface<lsx::kDivFma>(ring[0], ring[1], ring[2], ring[3], ring[4]);
+        fy1 = sface<lsx::kDivFma>(ring[1], ring[2], ring[3], ring[4], ring[5]);
+    } else {
+        fy0 = sface<lsx::kDivIEEE>(ring[0], ring[1], ring[2], ring[3], ring[4]);
+        fy1 = sface<lsx::kDivIEEE>(ring[1], ring[2], ring[3], ring[4], ring[5]);
+    }
     double cu = 0.0, cv = 0.0, cgn = 0.0, xh = 0.0;
     if (cell) {
         const int64_t c = (int64_t)r0 * s + X;
@@ -177,6 +199,8 @@ __global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int n
     for (int r = r0; r < r1; r++) {
         L.g[l + 2] = ring[3];
         if (xl) L.g[xp] = xh;
+        bad |= !lsx::crdiv_safe(xh);
+        const bool fd = fd_ok && __ballot(bad) == 0;  // this row's quotients by the FMA sequences
         double nr = 0.0, nu = 0.0, nv = 0.0, ngn = 0.0, nxh = 0.0;
         if (r + 1 < r1) {
             if (col) nr = Gs[(int64_t)(r + 4) * s + X];
@@ -189,8 +213,14 @@ __global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int n
             if (xl && xok) nxh = Gs[(int64_t)(r + 1) * s + xs + xp];
         }
         __syncthreads();  // row in LDS
-        L.f[l] = sface(L.g[l], L.g[l + 1], L.g[l + 2], L.g[l + 3], L.g[l + 4]);
-        const double fy2 = sface(ring[2], ring[3], ring[4], ring[5], ring[6]);  // y face r+3/2
+        double fy2;  // y face r+3/2
+        if (fd) {
+            L.f[l] = sface<lsx::kDivFma>(L.g[l], L.g[l + 1], L.g[l + 2], L.g[l + 3], L.g[l + 4]);
+            fy2 = sface<lsx::kDivFma>(ring[2], ring[3], ring[4], ring[5], ring[6]);
+        } else {
+            L.f[l] = sface<lsx::kDivIEEE>(L.g[l], L.g[l + 1], L.g[l + 2], L.g[l + 3], L.g[l + 4]);
+            fy2 = sface<lsx::kDivIEEE>(ring[2], ring[3], ring[4], ring[5], ring[6]);
+        }
         __syncthreads();  // x faces
         if (cell) {
             const int64_t c = (int64_t)r * s + X;
@@ -200,7 +230,8 @@ __global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int n
             } else if (r == g.ny + 2) {
                 val = 2.0 * o1 - o0;
             } else {
-                const double rr = srhs_f(g, L.f[l - 1], L.f[l], L.f[l + 1], fy0, fy1, fy2, cu, cv, S_L);
+                const double rr = fd ? srhs_f<lsx::kDivFma>(g, L.f[l - 1], L.f[l], L.f[l + 1], fy0, fy1, fy2, cu, cv, S_L)
+                                     : srhs_f<lsx::kDivIEEE>(g, L.f[l - 1], L.f[l], L.f[l + 1], fy0, fy1, fy2, cu, cv, S_L);
                 val = scombine<STAGE>(same ? ring[3] : cgn, ring[3], rr, dt);
             }
             const int nj = s_nimg(r, g.ny), ni = s_nimg(X, g.nx);
@@ -212,6 +243,7 @@ __global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int n
 #pragma unroll
         for (int q = 0; q < 6; q++) ring[q] = ring[q + 1];
         ring[6] = nr;
+        bad |= !lsx::crdiv_safe(nr);
         fy0 = fy1;
         fy1 = fy2;
         cu = nu;
@@ -438,6 +470,9 @@ int ls_sample_create(int32_t nx, int32_t ny, double dx, double dy, ls_sample_t**
     s->g.nyt = ny + 6;
     s->g.dx = dx;
     s->g.dy = dy;
+    s->g.rdx = 1.0 / dx;
+    s->g.rdy = 1.0 / dy;
+    s->g.crdiv = lsx::crdiv_geom_ok(dx, dy, 1.0, 1e-14);  // 1e-14: the eps under the |grad G| of the LLF speeds
     s->n = (int64_t)s->g.nxt * s->g.nyt;
     const size_t fb = (size_t)s->n * sizeof(double);
     if (hipMalloc((void**)&s->G1, fb) != hipSuccess || hipMalloc((void**)&s->G2, fb) != hipSuccess) {

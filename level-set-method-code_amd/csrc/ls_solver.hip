@@ -269,6 +269,9 @@ Geom make_geom(const ls_params_t& p) {
     g.dx = p.dx;
     g.dy = p.dy;
     g.dz = p.dim == 3 ? p.dz : 1.0;
+    g.rdx = 1.0 / g.dx;
+    g.rdy = 1.0 / g.dy;
+    g.rdz = 1.0 / g.dz;
     g.s_l = p.s_l;
     g.eps = p.eps;
     g.dtau = p.reinit_dtau;

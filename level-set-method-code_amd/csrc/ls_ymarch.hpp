@@ -18,6 +18,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ls_api.h"
 #include "ls_exact.hpp"
 #include "ls_fast_math.hpp"
@@ -49,6 +51,9 @@ struct YGrid {
     double s_l, eps, dt;
     double inv2dx, inv2dy, inv6dx, inv6dy;
     double dx, dy;        // exact mode divides by these, as the reference does
+    double rdx, rdy;      // RN(1/dx), RN(1/dy) for lsx::qdiv_u
+    int32_t crdiv;        // spacings, eps and a scalar S_L admit the FMA division sequences
+    int32_t pad2;
 };
 
 // Rows the y-march loads ahead of the one it computes (1 or 2).
@@ -158,7 +163,11 @@ __device__ __forceinline__ void ycell(const YGrid& m, int i, int j, const double
 }
 
 // Rows [y0, y1) of the strip that starts at column x0.  Called by every lane
-// of a one-wavefront block (blockDim.x == 64) with the same arguments.
+// of a one-wavefront block (blockDim.x == 64) with the same arguments.  In
+// the exact mode the divisions run as the FMA sequences of ls_crdiv.hpp while
+// every G and S_L value the wavefront has loaded passed lsx::crdiv_safe (a
+// sticky per-lane flag and a wave vote) and m.crdiv is set; otherwise as
+// divisions.  Both give the same bits.
 template <int STAGE, bool EXACT = false>
 __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int y1,
                                              const double* __restrict__ Gs, const double* __restrict__ Gn,
@@ -181,6 +190,10 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
     double r[7];  // G(i, j-3 .. j+3)
 #pragma unroll
     for (int q = 0; q < 7; q++) r[q] = act ? Gs[col + prow(y0 - 3 + q)] : 0.0;
+    bool bad = false;  // exact mode: a value this lane loaded failed crdiv_safe
+    if (EXACT)
+#pragma unroll
+        for (int q = 0; q < 7; q++) bad |= !lsx::crdiv_safe(r[q]);
     double ty[5];  // T(i, j-2 .. j+2) along y (fast mode)
 #pragma unroll
     for (int q = 0; q < 5; q++) ty[q] = EXACT ? 0.0 : tsq(r[q], r[q + 1], r[q + 2]);
@@ -212,6 +225,11 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         // --- the row into LDS; prefetch
         if (act) L.g[3 + l] = r[3];
         if (hl) L.g[hp] = cur.hal;
+        bool fd = false;  // exact mode: this row's quotients by the FMA sequences
+        if (EXACT) {
+            bad |= !lsx::crdiv_safe(cur.hal) || (sl && !lsx::crdiv_safe(cur.sl));
+            fd = m.crdiv && __ballot(bad) == 0;
+        }
         const YRow fresh = fetch(j + LS_YMARCH_PF, true);
         __syncthreads();  // row
         // --- smoothness terms along x, u_eff, orientations
@@ -224,14 +242,20 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         }
         double ue = cur.u, ve = cur.v;
         if (act && EXACT) {  // computeRHS (rk3.h:54-98), the reference's expression order
-            double gx = 0.0, gy = 0.0, gm = 1.0;
-            if (cur.sl > m.eps) {
-                gx = (L.g[4 + l] - L.g[2 + l]) / (2.0 * m.dx);
-                gy = (r[4] - r[2]) / (2.0 * m.dy);
-                gm = sqrt(gx * gx + gy * gy + m.eps);
-            }
-            ue = cur.u - cur.sl * gx / gm;
-            ve = cur.v - cur.sl * gy / gm;
+            auto ueff = [&](auto dtag) {
+                constexpr int D = decltype(dtag)::value;
+                double gx = 0.0, gy = 0.0, gm = 1.0;
+                if (cur.sl > m.eps) {  // 2h is exact, and RN(1/(2h)) = RN(1/h) / 2
+                    gx = lsx::dq<D>(L.g[4 + l] - L.g[2 + l], 2.0 * m.dx, 0.5 * m.rdx);
+                    gy = lsx::dq<D>(r[4] - r[2], 2.0 * m.dy, 0.5 * m.rdy);
+                    gm = sqrt(gx * gx + gy * gy + m.eps);
+                }
+                const double ygm = D == lsx::kDivIEEE ? 0.0 : 1.0 / gm;
+                ue = cur.u - lsx::dq<D>(cur.sl * gx, gm, ygm);
+                ve = cur.v - lsx::dq<D>(cur.sl * gy, gm, ygm);
+            };
+            if (fd) ueff(std::integral_constant<int, lsx::kDivFma>());
+            else ueff(std::integral_constant<int, lsx::kDivIEEE>());
         } else if (act) {
             yueff(m, L.g[2 + l], L.g[4 + l], r[2], r[4], cur.sl, ue, ve);
         }
@@ -250,12 +274,19 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
                 nR = nR || !o;
             }
             if (EXACT) {  // weno5_left / weno5_right values (weno5.h:62-122); .x holds the value
-                const double* g = &L.g[3 + l];
-                if (nL) L.f[0][l + 1].x = lsx::weno_l(g[-2], g[-1], g[0], g[1], g[2]);
-                if (nR) L.f[1][l + 1].x = lsx::weno_r(g[-1], g[0], g[1], g[2], g[3]);
-                if (ylo_L != (int)Ly)
-                    ylo.n = Ly ? lsx::weno_l(r[0], r[1], r[2], r[3], r[4]) : lsx::weno_r(r[1], r[2], r[3], r[4], r[5]);
-                yhi.n = Ly ? lsx::weno_l(r[1], r[2], r[3], r[4], r[5]) : lsx::weno_r(r[2], r[3], r[4], r[5], r[6]);
+                auto faces = [&](auto dtag) {
+                    constexpr int D = decltype(dtag)::value;
+                    const double* g = &L.g[3 + l];
+                    if (nL) L.f[0][l + 1].x = lsx::weno_l<D>(g[-2], g[-1], g[0], g[1], g[2]);
+                    if (nR) L.f[1][l + 1].x = lsx::weno_r<D>(g[-1], g[0], g[1], g[2], g[3]);
+                    if (ylo_L != (int)Ly)
+                        ylo.n = Ly ? lsx::weno_l<D>(r[0], r[1], r[2], r[3], r[4])
+                                   : lsx::weno_r<D>(r[1], r[2], r[3], r[4], r[5]);
+                    yhi.n = Ly ? lsx::weno_l<D>(r[1], r[2], r[3], r[4], r[5])
+                               : lsx::weno_r<D>(r[2], r[3], r[4], r[5], r[6]);
+                };
+                if (fd) faces(std::integral_constant<int, lsx::kDivFma>());
+                else faces(std::integral_constant<int, lsx::kDivIEEE>());
             } else {
                 face_put<1, 1, kYW + 1>(&L.g[3 + l], &L.t[3 + l], nL, nR, &L.f[0][l + 1]);
                 // y face j-1/2 in this cell's orientation (carried unless it differs), and j+1/2
@@ -268,9 +299,12 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
             }
         }
         if (l == 0) {  // x face x0-1/2 in the orientation of the strip's first cell
-            if (EXACT)
-                L.f[Lx ? 0 : 1][0].x = Lx ? lsx::weno_l(L.g[0], L.g[1], L.g[2], L.g[3], L.g[4])
-                                          : lsx::weno_r(L.g[1], L.g[2], L.g[3], L.g[4], L.g[5]);
+            if (EXACT && fd)
+                L.f[Lx ? 0 : 1][0].x = Lx ? lsx::weno_l<lsx::kDivFma>(L.g[0], L.g[1], L.g[2], L.g[3], L.g[4])
+                                          : lsx::weno_r<lsx::kDivFma>(L.g[1], L.g[2], L.g[3], L.g[4], L.g[5]);
+            else if (EXACT)
+                L.f[Lx ? 0 : 1][0].x = Lx ? lsx::weno_l<lsx::kDivIEEE>(L.g[0], L.g[1], L.g[2], L.g[3], L.g[4])
+                                          : lsx::weno_r<lsx::kDivIEEE>(L.g[1], L.g[2], L.g[3], L.g[4], L.g[5]);
             else
                 face_put<1, 1, kYW + 1>(&L.g[2], &L.t[2], Lx, !Lx, &L.f[0][0]);
         }
@@ -281,8 +315,9 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
             const double2 fl = L.f[o][l], fh = L.f[o][l + 1];
             double o2;
             if (EXACT) {  // (F(i+1/2) - F(i-1/2)) / h, rhs and the stage in the reference's order
-                const double dgx = (fh.x - fl.x) / m.dx;
-                const double dgy = (yhi.n - ylo.n) / m.dy;
+                const double ax = fh.x - fl.x, ay = yhi.n - ylo.n;
+                const double dgx = fd ? lsx::qdiv_u(ax, m.dx, m.rdx) : ax / m.dx;
+                const double dgy = fd ? lsx::qdiv_u(ay, m.dy, m.rdy) : ay / m.dy;
                 const double rr = -(ue * dgx + ve * dgy);
                 o2 = lsx::stage_combine<STAGE>(same ? r[3] : cur.gn, r[3], rr, m.dt);
             } else {  // both axes over one denominator
@@ -309,6 +344,7 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
             cur = fresh;
         }
         r[6] = cur.g;
+        if (EXACT) bad |= !lsx::crdiv_safe(r[6]);
 #pragma unroll
         for (int q = 0; q < 4; q++) ty[q] = ty[q + 1];
         ty[4] = EXACT ? 0.0 : tsq(r[4], r[5], r[6]);

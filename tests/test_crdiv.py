@@ -1,0 +1,60 @@
+"""The exact mode's FMA division sequences (ls_crdiv.hpp) against the IEEE
+division, on the host.
+
+The device kernels include the same header, so these are the sequences the
+exact-mode GPU kernels run; the GPU parity tests then check the kernels bit
+for bit against the oracle, which divides.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(HERE), "level-set-method-code_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path_factory.mktemp("crdiv") / "crdiv_check")
+    # -mfma: the host fma() is then the FMA instruction (it is correctly rounded either way)
+    subprocess.run([gxx, "-O2", "-mfma", "-ffp-contract=off", "-std=c++17", "-I", CSRC,
+                    os.path.join(HERE, "crdiv_check.cpp"), "-o", exe, "-lm"], check=True)
+    return exe
+
+
+def _run(exe, *args):
+    out = subprocess.run([exe, *args], check=True, stdout=subprocess.PIPE, text=True).stdout.split()
+    return dict(div6=int(out[1]), qdiv=int(out[3]), cases=int(out[5]))
+
+
+def test_sequences_match_division(checker):
+    """~60M quotients: random operands over +-400 binades, quotients within
+    2^-106 ulp of a rounding midpoint, exact quotients, signed zeros,
+    infinities, the range edges and the WENO-5 weight regime."""
+    r = _run(checker, "4000000")
+    assert r["cases"] > 55_000_000
+    assert r["div6"] == 0 and r["qdiv"] == 0, r
+
+
+def test_cases_catch_uncorrected_quotients(checker):
+    """Without the FMA corrections (x * RN(1/6), a * RN(1/b)) the same cases
+    find wrongly rounded quotients."""
+    r = _run(checker, "100000", "weak")
+    assert r["div6"] > 0 and r["qdiv"] > 0, r
+
+
+def test_safe_values_keep_quotients_in_domain(checker):
+    """The marching kernels use the unchecked sequences while every G and S_L
+    value passes crdiv_safe (0 or a magnitude in [2^-200, 2^50]) and the
+    spacings lie in [2^-60, 2^60].  On random stencils of such values,
+    extremes included, every quotient of the WENO-5 faces, the derivatives
+    and the effective velocities (reference solver and sample scheme) falls
+    inside the domain the sequences are proven for."""
+    out = subprocess.run([checker, "2000000", "range"], check=True, stdout=subprocess.PIPE, text=True).stdout.split()
+    assert out[0] == "range" and int(out[3]) >= 80_000_000
+    assert int(out[1]) == 0, out

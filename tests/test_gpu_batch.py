@@ -95,6 +95,32 @@ def test_batch_exact_all_shapes_bitwise(ls, oracle, cuda):
             assert np.array_equal(b.view(Gs, q).cpu().numpy(), refs[q]), "shape %d member %d" % (shape, q)
 
 
+def test_batch_exact_outside_fma_division_class_bitwise(ls, oracle, cuda):
+    """One member with planted values outside the class for which the exact
+    y-march divides with FMA sequences (|G| above 2^50, nonzero |G| below
+    2^-200): its wavefronts that load them take the IEEE divisions; every
+    member stays bitwise equal to the oracle."""
+    b, G, U, V, inputs = _build(ls, cuda, ls.MODE_EXACT)
+    q = 3
+    g, Gh, uh, vh, dt = inputs[q]
+    rng = np.random.default_rng(8)
+    Gp = Gh.copy()
+    flat = Gp.reshape(-1)
+    idx = rng.choice(flat.size, size=12, replace=False)
+    flat[idx[:6]] *= 1e20
+    flat[idx[6:]] = 1e-250
+    b.view(G, q).copy_(cuda.from_numpy(Gp))
+    b.fill_ghosts(G)
+    inputs[q] = (g, b.view(G, q).cpu().numpy(), uh, vh, dt)
+    steps = 3
+    b.advance(G, U, V, steps, graph=False)
+    cuda.cuda.synchronize()
+    for m, (gm, Gm, um, vm, dtm) in enumerate(inputs):
+        ref = oracle.advance(gm, Gm, um, vm, dt=dtm, t_final=1e30, max_steps=steps)[0]
+        assert np.isfinite(ref).all()
+        assert np.array_equal(b.view(G, m).cpu().numpy(), ref), "member %d" % m
+
+
 def test_batch_autotune_keeps_results(ls, cuda):
     b, G, U, V, _ = _build(ls, cuda, ls.MODE_EXACT)
     G2 = G.clone()

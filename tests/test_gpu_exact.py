@@ -191,3 +191,57 @@ def test_deformation_velocity_in_loop(ls, oracle, cuda):
     ref, n, _, _ = oracle.advance(g, G, u, v, w, dt=0.01, t_final=0.2, deform_T=1.5)
     assert rel_linf(du.cpu().numpy(), u) < 1e-14
     assert rel_linf(dG.cpu().numpy(), ref) < 1e-12
+
+
+def _plant_unsafe(G, seed):
+    """A few values outside the class for which the stage kernels divide with
+    FMA sequences (|G| above 2^50 or nonzero below 2^-200): the blocks and
+    wavefronts that load them take the IEEE divisions, the others do not."""
+    rng = np.random.default_rng(seed)
+    flat = G.reshape(-1)
+    idx = rng.choice(flat.size, size=max(4, flat.size // 2000), replace=False)
+    h = len(idx) // 2
+    flat[idx[:h]] *= 1e20
+    flat[idx[h:]] = np.where(flat[idx[h:]] < 0, -1e-250, 1e-250)
+    return G
+
+
+@pytest.mark.parametrize("dim,n,bc", [CASES[0], CASES[5], CASES[8]])
+def test_rk3_step_outside_fma_division_class_bitwise(ls, oracle, cuda, dim, n, bc):
+    """Planted G and S_L values outside the FMA-division class: the step is
+    still bitwise equal to the oracle (both division paths give the same
+    bits, and the planted blocks must take the IEEE one)."""
+    print("exact unsafe-class", dim, n, bc, flush=True)
+    g, p = _grid_and_params(ls, dim, n, bc, 0.0)
+    G, vel = _inputs(g, seed=17)
+    G = oracle.fill_ghosts(g, _plant_unsafe(G, 3))
+    rng = np.random.default_rng(4)
+    sl = rng.uniform(0.0, 0.2, g.shape)
+    sl.reshape(-1)[rng.choice(sl.size, size=max(2, sl.size // 3000), replace=False)] = 1e20
+    s = ls.Solver(p)
+    out = s.empty()
+    w = vel[2] if dim == 3 else None
+    s.rk3_step(_dev(cuda, G), out, _dev(cuda, vel[0]), _dev(cuda, vel[1]), _dev(cuda, w), dt=1e-24,
+               sl=_dev(cuda, sl))
+    ref = oracle.rk3_step(g, G, vel[0], vel[1], w, dt=1e-24, sl=sl)
+    got = out.cpu().numpy()
+    assert np.isfinite(ref).all()
+    assert np.array_equal(got, ref), "rel Linf %.3e" % rel_linf(got, ref)
+
+
+def test_rk3_step_tiny_spacing_bitwise(ls, oracle, cuda):
+    """Grid spacing below 2^-60: the host keeps every stage quotient on the
+    IEEE divisions."""
+    dim, n, bc = CASES[5]
+    g, p = _grid_and_params(ls, dim, n, bc, 0.1)
+    scale = 2.0 ** -70
+    g.dx, g.dy, g.dz = g.dx * scale, g.dy * scale, g.dz * scale
+    p.dx, p.dy, p.dz = g.dx, g.dy, g.dz
+    G, vel = _inputs(g, seed=19)
+    G = oracle.fill_ghosts(g, G * scale)
+    s = ls.Solver(p)
+    out = s.empty()
+    dt = 0.3 * g.dx
+    s.rk3_step(_dev(cuda, G), out, _dev(cuda, vel[0]), _dev(cuda, vel[1]), _dev(cuda, vel[2]), dt=dt)
+    ref = oracle.rk3_step(g, G, *vel, dt=dt)
+    assert np.array_equal(out.cpu().numpy(), ref)

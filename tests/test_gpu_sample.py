@@ -44,6 +44,32 @@ def test_sample_rhs_and_rk3_bitwise(ls, cuda, nx, ny, S_L):
     s.close()
 
 
+@pytest.mark.parametrize("S_L", [0.4, 1e20])
+def test_sample_rk3_outside_fma_division_class_bitwise(ls, cuda, S_L):
+    """Planted G values outside the class for which the sample y-march
+    divides with FMA sequences (|G| above 2^50, nonzero |G| below 2^-200),
+    and an S_L outside it: the wavefronts concerned take the IEEE divisions
+    and the step stays bitwise equal to the oracle."""
+    torch = cuda
+    print("sample unsafe-class", S_L, flush=True)
+    nx, ny = 130, 66
+    dx, dy, G, G0, u, v = _inputs(nx, ny, 5)
+    rng = np.random.default_rng(6)
+    flat = G.reshape(-1)
+    idx = rng.choice(flat.size, size=16, replace=False)
+    flat[idx[:8]] *= 1e20
+    flat[idx[8:]] = 1e-250
+    O = SampleOracle()
+    s = ls.Sample(nx, ny, dx, dy)
+    d = lambda a: torch.from_numpy(a).cuda()
+    Gd = d(G)
+    s.rk3_step(Gd, d(u), d(v), 2.5e-6, S_L, d(G0))
+    ref = O.rk3_step(G, u, v, dx, dy, 2.5e-6, S_L, G0)
+    assert np.isfinite(ref).all()
+    assert np.array_equal(Gd.cpu().numpy(), ref)
+    s.close()
+
+
 @pytest.mark.parametrize("nx,ny", SHAPES)
 def test_sample_boundaries_reinit_first_order_bitwise(ls, cuda, nx, ny):
     torch = cuda
