@@ -25,7 +25,8 @@
 //
 // Both arguments need every intermediate in the normal range; the _u forms
 // state operand ranges that guarantee it and do not check them.  div6 and
-// qdiv check those ranges and take the division outside them.  The
+// qdiv check (a sufficient part of) those ranges and take the division
+// outside them; the host tests run the sequences through them.  The
 // corrections are written q + (b q - a)(-y), not q + (a - b q) y, so that a
 // zero numerator keeps its sign as the division does (b q - a is +0 for a
 // zero a, and +0 times a negative factor is -0); the divisor must be > 0.
@@ -98,21 +99,19 @@ LS_CRDIV_FN bool crdiv_geom_ok(double dx, double dy, double dz, double eps) {
            eps >= 0x1p-200 && eps <= 0x1p50;
 }
 
-// Division flavours of the exact-mode helpers: the IEEE division itself, the
-// checked sequences, or the unchecked ones (caller guarantees the ranges).
-enum : int { kDivIEEE = 0, kDivChecked = 1, kDivFma = 2 };
+// Division flavours of the exact-mode helpers: the IEEE division itself, or
+// the unchecked sequences (the caller guarantees their ranges).
+enum : int { kDivIEEE = 0, kDivFma = 1 };
 
 template <int D>
 LS_CRDIV_FN double d6(double x) {
     if constexpr (D == kDivIEEE) return x / 6.0;
-    else if constexpr (D == kDivChecked) return div6(x);
     else return div6_u(x);
 }
 
 template <int D>
 LS_CRDIV_FN double dq(double a, double b, double y) {
     if constexpr (D == kDivIEEE) return a / b;
-    else if constexpr (D == kDivChecked) return qdiv(a, b, y);
     else return qdiv_u(a, b, y);
 }
 

@@ -3,9 +3,10 @@
 // Every expression keeps the reference's C++ evaluation order and is compiled
 // with -ffp-contract=off, so that each cell reproduces the contraction-free
 // CPU result bit for bit (IEEE double, correctly rounded division and sqrt).
-// Divisions by 6 and by a divisor shared between several quotients (the
-// weight sum, the grid spacing, |grad G|) may go through the FMA sequences of
-// ls_crdiv.hpp, which return the same correctly rounded quotients.
+// The marching stage kernels may carry out the divisions by 6 and by a
+// divisor shared between several quotients (the weight sum, the grid
+// spacing, |grad G|) with the FMA sequences of ls_crdiv.hpp, which return the
+// same correctly rounded quotients; the per-cell kernels here divide.
 //
 //   smoothness indicators  Local_code/level-set_MPI_3D/include/weno5.h:28-37
 //   weno5_left / right     Local_code/level-set_MPI_3D/include/weno5.h:39-81
@@ -31,7 +32,7 @@ struct Geom {
     int32_t bc[3];
     int64_t sy, sz;      // strides of y and z in doubles
     double dx, dy, dz;
-    double rdx, rdy, rdz;  // RN(1/dx), RN(1/dy), RN(1/dz) for qdiv
+    double rdx, rdy, rdz;  // RN(1/dx), RN(1/dy), RN(1/dz) for the marching kernels' lsx::qdiv_u
     double s_l, eps;
     double dtau, beta;
 };
@@ -45,7 +46,7 @@ __device__ __forceinline__ double beta3(double p, double q, double r) {
 // weno5_left on (v0..v4): reconstruction at the right face of v2.  D selects
 // how the divisions are carried out (ls_crdiv.hpp); every choice returns the
 // same correctly rounded quotients.
-template <int D = kDivChecked>
+template <int D = kDivIEEE>
 __device__ __forceinline__ double weno_l(double v0, double v1, double v2, double v3, double v4) {
     double b0 = beta3(v0, v1, v2);
     double b1 = beta3(v1, v2, v3);
@@ -64,7 +65,7 @@ __device__ __forceinline__ double weno_l(double v0, double v1, double v2, double
 
 // weno5_right on (v0..v4): the indicators are taken on the mirrored stencil
 // (v4,v3,v2,v1,v0), so beta_0 uses (v4,v3,v2), beta_1 (v3,v2,v1), beta_2 (v2,v1,v0).
-template <int D = kDivChecked>
+template <int D = kDivIEEE>
 __device__ __forceinline__ double weno_r(double v0, double v1, double v2, double v3, double v4) {
     double b0 = beta3(v4, v3, v2);
     double b1 = beta3(v3, v2, v1);
@@ -81,17 +82,17 @@ __device__ __forceinline__ double weno_r(double v0, double v1, double v2, double
     return w0 * p0 + w1 * p1 + w2 * p2;
 }
 
-// Upwinded WENO-5 derivative from the 7-point axis stencil m3..p3; rh = RN(1/h).
+// Upwinded WENO-5 derivative from the 7-point axis stencil m3..p3.
 __device__ __forceinline__ double dweno(double m3, double m2, double m1, double c0, double p1,
-                                        double p2, double p3, double vel, double h, double rh) {
+                                        double p2, double p3, double vel, double h) {
     if (vel >= 0.0) {
-        double gl = weno_l(m3, m2, m1, c0, p1);
-        double gr = weno_l(m2, m1, c0, p1, p2);
-        return qdiv(gr - gl, h, rh);
+        double gl = weno_l<kDivIEEE>(m3, m2, m1, c0, p1);
+        double gr = weno_l<kDivIEEE>(m2, m1, c0, p1, p2);
+        return (gr - gl) / h;
     }
-    double gl = weno_r(m2, m1, c0, p1, p2);
-    double gr = weno_r(m1, c0, p1, p2, p3);
-    return qdiv(gr - gl, h, rh);
+    double gl = weno_r<kDivIEEE>(m2, m1, c0, p1, p2);
+    double gr = weno_r<kDivIEEE>(m1, c0, p1, p2, p3);
+    return (gr - gl) / h;
 }
 
 // RHS -(u_eff . grad_w G) at the cell P points to.
@@ -101,24 +102,22 @@ __device__ __forceinline__ double rhs_cell(const Geom& g, const double* __restri
     const int64_t sy = g.sy, sz = g.sz;
     double gx = 0.0, gy = 0.0, gz = 0.0, gm = 1.0;
     if (s_l > g.eps) {
-        // 2h is exact, and RN(1/(2h)) = RN(1/h) / 2
-        gx = qdiv(P[1] - P[-1], 2.0 * g.dx, 0.5 * g.rdx);
-        gy = qdiv(P[sy] - P[-sy], 2.0 * g.dy, 0.5 * g.rdy);
+        gx = (P[1] - P[-1]) / (2.0 * g.dx);
+        gy = (P[sy] - P[-sy]) / (2.0 * g.dy);
         if (DIM == 3) {
-            gz = qdiv(P[sz] - P[-sz], 2.0 * g.dz, 0.5 * g.rdz);
+            gz = (P[sz] - P[-sz]) / (2.0 * g.dz);
             gm = sqrt(gx * gx + gy * gy + gz * gz + g.eps);
         } else {
             gm = sqrt(gx * gx + gy * gy + g.eps);
         }
     }
-    const double ygm = 1.0 / gm;
-    double ue = uc - qdiv(s_l * gx, gm, ygm);
-    double ve = vc - qdiv(s_l * gy, gm, ygm);
-    double dgx = dweno(P[-3], P[-2], P[-1], P[0], P[1], P[2], P[3], ue, g.dx, g.rdx);
-    double dgy = dweno(P[-3 * sy], P[-2 * sy], P[-sy], P[0], P[sy], P[2 * sy], P[3 * sy], ve, g.dy, g.rdy);
+    double ue = uc - s_l * gx / gm;
+    double ve = vc - s_l * gy / gm;
+    double dgx = dweno(P[-3], P[-2], P[-1], P[0], P[1], P[2], P[3], ue, g.dx);
+    double dgy = dweno(P[-3 * sy], P[-2 * sy], P[-sy], P[0], P[sy], P[2 * sy], P[3 * sy], ve, g.dy);
     if (DIM == 3) {
-        double we = wc - qdiv(s_l * gz, gm, ygm);
-        double dgz = dweno(P[-3 * sz], P[-2 * sz], P[-sz], P[0], P[sz], P[2 * sz], P[3 * sz], we, g.dz, g.rdz);
+        double we = wc - s_l * gz / gm;
+        double dgz = dweno(P[-3 * sz], P[-2 * sz], P[-sz], P[0], P[sz], P[2 * sz], P[3 * sz], we, g.dz);
         return -(ue * dgx + ve * dgy + we * dgz);
     }
     return -(ue * dgx + ve * dgy);
@@ -157,12 +156,12 @@ __device__ __forceinline__ double hcr2_cell(const Geom& g, const double* __restr
 
     const double phi = pn[0];
     const double S = pc / sqrt(pc * pc + g.dx * g.dx);
-    double xm = qdiv(pn[0] - pn[-1], g.dx, g.rdx), xp = qdiv(pn[1] - pn[0], g.dx, g.rdx);
-    double ym = qdiv(pn[0] - pn[-g.sy], g.dy, g.rdy), yp = qdiv(pn[g.sy] - pn[0], g.dy, g.rdy);
+    double xm = (pn[0] - pn[-1]) / g.dx, xp = (pn[1] - pn[0]) / g.dx;
+    double ym = (pn[0] - pn[-g.sy]) / g.dy, yp = (pn[g.sy] - pn[0]) / g.dy;
     double zm = 0.0, zp = 0.0;
     if (DIM == 3) {
-        zm = qdiv(pn[0] - pn[-g.sz], g.dz, g.rdz);
-        zp = qdiv(pn[g.sz] - pn[0], g.dz, g.rdz);
+        zm = (pn[0] - pn[-g.sz]) / g.dz;
+        zp = (pn[g.sz] - pn[0]) / g.dz;
     }
     double ax, bx, ay, by, az, bz;
     if (S > 0.0) {
@@ -190,7 +189,7 @@ __device__ __forceinline__ double hcr2_cell(const Geom& g, const double* __restr
 #pragma unroll
             for (int q = 0; q < NB; q++)
                 if ((flag >> q) & 1) sum += pn[off[q]];
-            forcing = qdiv(g.beta * (rt * sum - phi), g.dx, g.rdx);
+            forcing = g.beta * (rt * sum - phi) / g.dx;
         }
     }
     return phi + g.dtau * (term + forcing);

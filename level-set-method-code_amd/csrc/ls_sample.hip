@@ -57,7 +57,7 @@ __device__ __forceinline__ double sbeta(double f0, double f1, double f2) {
 
 // weno5_left of the reference (sample/weno5_rk3.cpp:17-32); D is the division
 // flavour (ls_crdiv.hpp), every choice giving the same bits.
-template <int D = lsx::kDivChecked>
+template <int D = lsx::kDivIEEE>
 __device__ __forceinline__ double sface(double v0, double v1, double v2, double v3, double v4) {
     const double b0 = sbeta(v0, v1, v2);
     const double b1 = sbeta(v1, v2, v3);
@@ -75,7 +75,7 @@ __device__ __forceinline__ double sface(double v0, double v1, double v2, double 
 // compute_geqn_rhs at one interior cell (sample/weno5_rk3.cpp:111-143) from
 // its three faces per axis, L at i-1/2, i+1/2, i+3/2 (dG_L and dG_R of
 // weno5_rk3.cpp:53-87 are their differences over h).
-template <int D = lsx::kDivChecked>
+template <int D = lsx::kDivIEEE>
 __device__ __forceinline__ double srhs_f(const SGeom& g, double fmx, double f0x, double fpx, double fmy, double f0y,
                                          double fpy, double uc, double vc, double S_L) {
     const double Gdxm = lsx::dq<D>(f0x - fmx, g.dx, g.rdx), Gdxp = lsx::dq<D>(fpx - f0x, g.dx, g.rdx);
