@@ -82,6 +82,36 @@ __device__ __forceinline__ double weno_r(double v0, double v1, double v2, double
     return w0 * p0 + w1 * p1 + w2 * p2;
 }
 
+// weno_l(u0..u4) when mirror is false; when mirror is true weno_r on the
+// stencil (u4, u3, u2, u1, u0), i.e. with its values given in mirrored order.
+// weno5_right is weno5_left's arithmetic on the mirrored stencil with the
+// linear weights (0.3, 0.6, 0.1), except that each candidate polynomial adds
+// its three terms in the opposite order (weno5.h:39-81).  Both orders are
+// selected per lane, so one call covers either orientation without a branch
+// and returns the bits of weno_l / weno_r.
+template <int D = kDivIEEE>
+__device__ __forceinline__ double weno_u(double u0, double u1, double u2, double u3, double u4, bool mirror) {
+    double b0 = beta3(u0, u1, u2);
+    double b1 = beta3(u1, u2, u3);
+    double b2 = beta3(u2, u3, u4);
+    const double d0 = mirror ? 0.3 : 0.1, d2 = mirror ? 0.1 : 0.3;
+    double a0 = d0 / ((kWenoEps + b0) * (kWenoEps + b0));
+    double a1 = 0.6 / ((kWenoEps + b1) * (kWenoEps + b1));
+    double a2 = d2 / ((kWenoEps + b2) * (kWenoEps + b2));
+    double s = a0 + a1 + a2;
+    const double ys = D == kDivIEEE ? 0.0 : 1.0 / s;
+    double w0 = dq<D>(a0, s, ys), w1 = dq<D>(a1, s, ys), w2 = dq<D>(a2, s, ys);
+    // left: p0 = (2u0 - 7u1) + 11u2, p1 = (-u1 + 5u2) + 2u3, p2 = (2u2 + 5u3) - u4;
+    // right: the first and last term of each swapped
+    const double e0 = 2.0 * u0, f0 = 11.0 * u2;
+    const double e1 = -u1, f1 = 2.0 * u3;
+    const double e2 = 2.0 * u2, f2 = -u4;
+    double p0 = d6<D>((mirror ? f0 : e0) - 7.0 * u1 + (mirror ? e0 : f0));
+    double p1 = d6<D>((mirror ? f1 : e1) + 5.0 * u2 + (mirror ? e1 : f1));
+    double p2 = d6<D>((mirror ? f2 : e2) + 5.0 * u3 + (mirror ? e2 : f2));
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
 // Upwinded WENO-5 derivative from the 7-point axis stencil m3..p3.
 __device__ __forceinline__ double dweno(double m3, double m2, double m1, double c0, double p1,
                                         double p2, double p3, double vel, double h) {

@@ -274,16 +274,26 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
                 nR = nR || !o;
             }
             if (EXACT) {  // weno5_left / weno5_right values (weno5.h:62-122); .x holds the value
+                // orientations are per-lane choices of the stencil order
+                // (lsx::weno_u), not branches: every lane builds the first x
+                // face it needs, lanes that need both then the mirrored one
                 auto faces = [&](auto dtag) {
                     constexpr int D = decltype(dtag)::value;
                     const double* g = &L.g[3 + l];
-                    if (nL) L.f[0][l + 1].x = lsx::weno_l<D>(g[-2], g[-1], g[0], g[1], g[2]);
-                    if (nR) L.f[1][l + 1].x = lsx::weno_r<D>(g[-1], g[0], g[1], g[2], g[3]);
+                    auto xf = [&](bool mir) {  // weno_l(g[-2..2]) or weno_r(g[-1..3])
+                        const double* b = mir ? g + 3 : g - 2;
+                        const int st = mir ? -1 : 1;
+                        return lsx::weno_u<D>(b[0], b[st], b[2 * st], b[3 * st], b[4 * st], mir);
+                    };
+                    const bool m1 = !nL;
+                    L.f[m1 ? 1 : 0][l + 1].x = xf(m1);
+                    if (nL && nR) L.f[1][l + 1].x = xf(true);
+                    const bool m = !Ly;
                     if (ylo_L != (int)Ly)
-                        ylo.n = Ly ? lsx::weno_l<D>(r[0], r[1], r[2], r[3], r[4])
-                                   : lsx::weno_r<D>(r[1], r[2], r[3], r[4], r[5]);
-                    yhi.n = Ly ? lsx::weno_l<D>(r[1], r[2], r[3], r[4], r[5])
-                               : lsx::weno_r<D>(r[2], r[3], r[4], r[5], r[6]);
+                        ylo.n = lsx::weno_u<D>(m ? r[5] : r[0], m ? r[4] : r[1], m ? r[3] : r[2], m ? r[2] : r[3],
+                                               m ? r[1] : r[4], m);
+                    yhi.n = lsx::weno_u<D>(m ? r[6] : r[1], m ? r[5] : r[2], m ? r[4] : r[3], m ? r[3] : r[4],
+                                           m ? r[2] : r[5], m);
                 };
                 if (fd) faces(std::integral_constant<int, lsx::kDivFma>());
                 else faces(std::integral_constant<int, lsx::kDivIEEE>());
@@ -299,12 +309,13 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
             }
         }
         if (l == 0) {  // x face x0-1/2 in the orientation of the strip's first cell
+            const bool m = !Lx;  // weno_l(L.g[0..4]) or weno_r(L.g[1..5])
             if (EXACT && fd)
-                L.f[Lx ? 0 : 1][0].x = Lx ? lsx::weno_l<lsx::kDivFma>(L.g[0], L.g[1], L.g[2], L.g[3], L.g[4])
-                                          : lsx::weno_r<lsx::kDivFma>(L.g[1], L.g[2], L.g[3], L.g[4], L.g[5]);
+                L.f[m ? 1 : 0][0].x = lsx::weno_u<lsx::kDivFma>(L.g[m ? 5 : 0], L.g[m ? 4 : 1], L.g[m ? 3 : 2],
+                                                                L.g[m ? 2 : 3], L.g[m ? 1 : 4], m);
             else if (EXACT)
-                L.f[Lx ? 0 : 1][0].x = Lx ? lsx::weno_l<lsx::kDivIEEE>(L.g[0], L.g[1], L.g[2], L.g[3], L.g[4])
-                                          : lsx::weno_r<lsx::kDivIEEE>(L.g[1], L.g[2], L.g[3], L.g[4], L.g[5]);
+                L.f[m ? 1 : 0][0].x = lsx::weno_u<lsx::kDivIEEE>(L.g[m ? 5 : 0], L.g[m ? 4 : 1], L.g[m ? 3 : 2],
+                                                                 L.g[m ? 2 : 3], L.g[m ? 1 : 4], m);
             else
                 face_put<1, 1, kYW + 1>(&L.g[2], &L.t[2], Lx, !Lx, &L.f[0][0]);
         }
