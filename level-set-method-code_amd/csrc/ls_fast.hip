@@ -348,16 +348,30 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
     struct Next {
         double lead, u, v, w, gn, sl, halo[NH];
     };
+    // The plane loads are buffer loads: a descriptor based at the plane
+    // (block-uniform, built in scalar registers) and a 32-bit byte offset per
+    // thread that stays fixed for the whole march, so no load needs
+    // per-thread 64-bit address arithmetic.  Plane offsets fit: sz < 2^29
+    // follows from the padded-size check at launch.
+    const uint32_t colb = (uint32_t)col * 8u;
+    uint32_t ohb[NH];
+#pragma unroll
+    for (int a = 0; a < NH; a++) ohb[a] = (uint32_t)ohalo[a] * 8u;
+    auto ld = [](const double* plane, uint32_t off) {
+        const __amdgpu_buffer_rsrc_t r =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(plane), (short)0, -1, 0x00020000);
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    };
     auto fetch = [&](int kk, Next& n) {  // plane kk's point data and halo values, and G(kk+3) of the column
         const int64_t pl = (int64_t)(kk + 3) * sz;
-        n.lead = Gs[pl + 3 * sz + col];
-        n.u = u[pl + col];
-        n.v = v[pl + col];
-        n.w = w[pl + col];
-        n.gn = same ? 0.0 : Gn[pl + col];  // stage 1 passes G_n as the stencil field
-        n.sl = sl ? sl[pl + col] : z.s_l;
+        n.lead = ld(Gs + pl + 3 * sz, colb);
+        n.u = ld(u + pl, colb);
+        n.v = ld(v + pl, colb);
+        n.w = ld(w + pl, colb);
+        n.gn = same ? 0.0 : ld(Gn + pl, colb);  // stage 1 passes G_n as the stencil field
+        n.sl = sl ? ld(sl + pl, colb) : z.s_l;
 #pragma unroll
-        for (int a = 0; a < NH; a++) n.halo[a] = Gs[pl + ohalo[a]];
+        for (int a = 0; a < NH; a++) n.halo[a] = ld(Gs + pl, ohb[a]);
     };
     auto put_plane = [&](double own, const Next& n) {
         sG[ly + 3][lx + 3] = own;
