@@ -197,20 +197,36 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
     double ty[5];  // T(i, j-2 .. j+2) along y (fast mode)
 #pragma unroll
     for (int q = 0; q < 5; q++) ty[q] = EXACT ? 0.0 : tsq(r[q], r[q + 1], r[q + 2]);
-    // point data of row jr and the ring's next value G(i, jr + 3)
+    // point data of row jr and the ring's next value G(i, jr + 3).  Fast mode
+    // loads them as buffer loads: a descriptor based at the row's padded x = 0
+    // (wavefront-uniform) and a 32-bit byte offset per lane fixed for the whole
+    // march.  (The exact variant keeps plain loads: its scalar registers are
+    // full, and the descriptors spilled them, 1.6% slower.)
+    const int64_t base = m.off;  // padded (0, 0) of this grid
+    const uint32_t cb = (uint32_t)(col - base) * 8u, hb = (uint32_t)(hcol - base) * 8u;
+    auto ld = [](const double* row, uint32_t off) {
+        if constexpr (EXACT) {
+            return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(row) + off);
+        } else {
+            const __amdgpu_buffer_rsrc_t r =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(row), (short)0, -1, 0x00020000);
+            return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+        }
+    };
     auto fetch = [&](int jr, bool with_g) {
         YRow q;
         q.g = q.u = q.v = q.gn = q.hal = 0.0;
         q.sl = m.s_l;
         if (jr < y1) {
+            const int64_t rj = base + prow(jr);
             if (act) {
-                if (with_g) q.g = Gs[col + prow(jr + 3)];
-                q.u = u[col + prow(jr)];
-                q.v = v[col + prow(jr)];
-                if (!same) q.gn = Gn[col + prow(jr)];
-                if (sl) q.sl = sl[col + prow(jr)];
+                if (with_g) q.g = ld(Gs + base + prow(jr + 3), cb);
+                q.u = ld(u + rj, cb);
+                q.v = ld(v + rj, cb);
+                if (!same) q.gn = ld(Gn + rj, cb);
+                if (sl) q.sl = ld(sl + rj, cb);
             }
-            if (hl) q.hal = Gs[hcol + prow(jr)];
+            if (hl) q.hal = ld(Gs + rj, hb);
         }
         return q;
     };
