@@ -245,3 +245,27 @@ def test_rk3_step_tiny_spacing_bitwise(ls, oracle, cuda):
     s.rk3_step(_dev(cuda, G), out, _dev(cuda, vel[0]), _dev(cuda, vel[1]), _dev(cuda, vel[2]), dt=dt)
     ref = oracle.rk3_step(g, G, *vel, dt=dt)
     assert np.array_equal(out.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("scale", [2.0 ** 40, 2.0 ** -150])
+@pytest.mark.parametrize("dim,n,bc", [CASES[5], CASES[8]])
+def test_rk3_step_extreme_magnitudes_bitwise(ls, oracle, cuda, dim, n, bc, scale):
+    """G (and the S_L field) scaled towards the ends of the range in which
+    the stage kernels divide with FMA sequences (|G| up to 2^50, down to
+    2^-200): the quotients then reach far into the sequences' proven domain,
+    and the step stays bitwise equal to the oracle."""
+    print("exact extreme magnitudes", dim, n, bc, scale, flush=True)
+    g, p = _grid_and_params(ls, dim, n, bc, 0.0)
+    G, vel = _inputs(g, seed=23)
+    G = oracle.fill_ghosts(g, G * scale)
+    rng = np.random.default_rng(12)
+    sl = rng.uniform(0.0, 0.2, g.shape) * scale
+    s = ls.Solver(p)
+    out = s.empty()
+    w = vel[2] if dim == 3 else None
+    dt = 0.3 * min(g.dx, g.dy)
+    s.rk3_step(_dev(cuda, G), out, _dev(cuda, vel[0]), _dev(cuda, vel[1]), _dev(cuda, w), dt=dt, sl=_dev(cuda, sl))
+    ref = oracle.rk3_step(g, G, vel[0], vel[1], w, dt=dt, sl=sl)
+    got = out.cpu().numpy()
+    assert np.isfinite(ref).all()
+    assert np.array_equal(got, ref), "rel Linf %.3e" % rel_linf(got, ref)
