@@ -3,9 +3,9 @@
 // exact mode / CPU oracle to rounding (tested at 1e-12 relative in G and in
 // zero-level-set position), not bit for bit.
 //
-// Per axis and cell the exact mode performs 2 WENO-5 reconstructions with 9
-// correctly rounded divisions each (Local_code/level-set_MPI_3D/include/weno5.h:39-81).
-// Here:
+// Per axis and cell the reference performs 2 WENO-5 reconstructions with 9
+// correctly rounded divisions each (Local_code/level-set_MPI_3D/include/weno5.h:39-81),
+// which the exact mode reproduces bit for bit.  Here:
 //  * the right-biased branch is the left-biased formula on the reversed
 //    stencil with the linear weights (0.1, 0.6, 0.3) swapped to (0.3, 0.6, 0.1)
 //    (this is the reference's weno5_right, weno5.h:60-81), so the upwind
