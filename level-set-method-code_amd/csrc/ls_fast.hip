@@ -507,13 +507,19 @@ __global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4,
             }
             if constexpr (EXACT) {
                 // z faces as single values in zlo.n
+                // weno_l(zG[q..q+4]) or weno_r(zG[q+1..q+5]), q = 0, 1, with the
+                // orientation a per-lane choice of the stencil order (lsx::weno_u):
+                // no branch when Lz differs within the wavefront.  (The x and y
+                // faces keep their branches: selecting their LDS stencils too
+                // raised the register count into spills.)
                 auto zfaces = [&](auto dtag) {
                     constexpr int D = decltype(dtag)::value;
+                    const bool m = !Lz;
                     if (zlo_L != (int)Lz)
-                        zlo.n = Lz ? lsx::weno_l<D>(zG[0], zG[1], zG[2], zG[3], zG[4])
-                                   : lsx::weno_r<D>(zG[1], zG[2], zG[3], zG[4], zG[5]);
-                    zhi.n = Lz ? lsx::weno_l<D>(zG[1], zG[2], zG[3], zG[4], zG[5])
-                               : lsx::weno_r<D>(zG[2], zG[3], zG[4], zG[5], zG[6]);
+                        zlo.n = lsx::weno_u<D>(m ? zG[5] : zG[0], m ? zG[4] : zG[1], m ? zG[3] : zG[2],
+                                               m ? zG[2] : zG[3], m ? zG[1] : zG[4], m);
+                    zhi.n = lsx::weno_u<D>(m ? zG[6] : zG[1], m ? zG[5] : zG[2], m ? zG[4] : zG[3],
+                                           m ? zG[3] : zG[4], m ? zG[2] : zG[5], m);
                 };
                 if (fd) zfaces(DivFma());
                 else zfaces(DivIEEE());
