@@ -239,21 +239,28 @@ __device__ __forceinline__ void store_images(const ZGeom& z, double* __restrict_
     }
 }
 
-// Tile of the shared-face kernel: 64 x FY cells per plane, one cell per
+// Tile of the shared-face kernel: 64 x rows cells per plane, one cell per
 // thread, one wave per row.  Besides its row, a wave may take a share of the
 // tile's edge work: waves 0-3 the y-edge smoothness terms (one row each),
 // wave kWaveTx the x-edge terms, waves kWaveFx and kWaveFy the x and y faces
-// on the low tile edges.
+// on the low tile edges.  The edge faces cost two wave passes per plane
+// whatever the height, so the exact mode, whose faces are several times
+// dearer, uses 8 rows (3.62 -> 3.58 ms at 512^3); the fast mode keeps 4
+// (8 rows: 1.78 -> 1.80 ms).
 #ifndef LS_ZFACE_ROWS
-#define LS_ZFACE_ROWS 4
+#define LS_ZFACE_ROWS 4  // fast mode
 #endif
-constexpr int FX = 64, FY = LS_ZFACE_ROWS;
-static_assert(FY == 4 || FY == 8, "tile rows");
-constexpr int kWaveTx = FY == 4 ? 2 : 5, kWaveFx = FY == 4 ? 0 : 6, kWaveFy = FY == 4 ? 1 : 7;
-constexpr int FGH = FY + 2 * ZH, FGW = FX + 2 * ZH;
-constexpr int kFaceThreads = FX * FY;
-constexpr int kHaloItems = 6 * FX + 6 * FY;   // G halo values per plane
-constexpr int NH = (kHaloItems + kFaceThreads - 1) / kFaceThreads;  // per thread
+constexpr int FX = 64, FGW = FX + 2 * ZH;  // LDS plane width: tile + 3-cell halo
+template <bool EXACT>
+struct ZTile {
+    static constexpr int FY = EXACT ? 8 : LS_ZFACE_ROWS;
+    static_assert(FY == 4 || FY == 8, "tile rows");
+    static constexpr int kWaveTx = FY == 4 ? 2 : 5, kWaveFx = FY == 4 ? 0 : 6, kWaveFy = FY == 4 ? 1 : 7;
+    static constexpr int FGH = FY + 2 * ZH;
+    static constexpr int kFaceThreads = FX * FY;
+    static constexpr int kHaloItems = 6 * FX + 6 * FY;   // G halo values per plane
+    static constexpr int NH = (kHaloItems + kFaceThreads - 1) / kFaceThreads;  // per thread
+};
 
 // A stored face as the value (exact mode) or the (n, d) pair (fast mode);
 // each overload is called only in its own mode.
@@ -286,10 +293,14 @@ using DivFma = std::integral_constant<int, lsx::kDivFma>;
 // the spacings in range (z.crdiv); otherwise as divisions.  Both give the
 // same bits, so waves may differ in which they use.
 template <int STAGE, bool EXACT>
-__global__ __launch_bounds__(kFaceThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_stage_zface(
+__global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_stage_zface(
     ZGeom z, const double* __restrict__ Gs, const double* __restrict__ Gn, double* __restrict__ out,
     const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
     const double* __restrict__ sl, double dt) {
+    using TT = ZTile<EXACT>;
+    constexpr int FY = TT::FY, kWaveTx = TT::kWaveTx, kWaveFx = TT::kWaveFx, kWaveFy = TT::kWaveFy;
+    constexpr int FGH = TT::FGH, kFaceThreads = TT::kFaceThreads, kHaloItems = TT::kHaloItems;
+    constexpr int NH = TT::NH;
     using FV = typename std::conditional<EXACT, double, double2>::type;  // a stored face
     __shared__ double sG[FGH][FGW];                      // plane k: tile + 3-cell halo
     __shared__ double sTx[EXACT ? 1 : FY][FX + 4];       // T along x at local x = -2..FX+1
@@ -927,6 +938,7 @@ static ZGeom make_z(const Geom& g, int kz, int tile_cols = ZX, int tile_rows = Z
 int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
                         const double* u, const double* v, const double* w, const double* sl, double dt,
                         hipStream_t st, bool exact) {
+    const int FY = exact ? ZTile<true>::FY : ZTile<false>::FY;
     const int kz = zmarch_chunk(g, FX, FY);
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
         return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
