@@ -245,17 +245,20 @@ __device__ __forceinline__ void store_images(const ZGeom& z, double* __restrict_
 // wave kWaveTx the x-edge terms, waves kWaveFx and kWaveFy the x and y faces
 // on the low tile edges.  The edge faces cost two wave passes per plane
 // whatever the height, so the exact mode, whose faces are several times
-// dearer, uses 8 rows (3.62 -> 3.58 ms at 512^3); the fast mode keeps 4
-// (8 rows: 1.78 -> 1.80 ms).
+// dearer, uses 8 rows (3.62 -> 3.58 ms at 512^3; 16 rows: 4.02 ms, from
+// barrier skew over 16 waves); the fast mode keeps 4 (8 rows: 1.78 -> 1.80 ms).
 #ifndef LS_ZFACE_ROWS
 #define LS_ZFACE_ROWS 4  // fast mode
+#endif
+#ifndef LS_ZFACE_ROWS_EXACT
+#define LS_ZFACE_ROWS_EXACT 8
 #endif
 constexpr int FX = 64, FGW = FX + 2 * ZH;  // LDS plane width: tile + 3-cell halo
 template <bool EXACT>
 struct ZTile {
-    static constexpr int FY = EXACT ? 8 : LS_ZFACE_ROWS;
-    static_assert(FY == 4 || FY == 8, "tile rows");
-    static constexpr int kWaveTx = FY == 4 ? 2 : 5, kWaveFx = FY == 4 ? 0 : 6, kWaveFy = FY == 4 ? 1 : 7;
+    static constexpr int FY = EXACT ? LS_ZFACE_ROWS_EXACT : LS_ZFACE_ROWS;
+    static_assert(FY == 4 || FY == 8 || FY == 16, "tile rows");
+    static constexpr int kWaveTx = FY == 4 ? 2 : 5, kWaveFx = FY == 4 ? 0 : FY - 2, kWaveFy = FY == 4 ? 1 : FY - 1;
     static constexpr int FGH = FY + 2 * ZH;
     static constexpr int kFaceThreads = FX * FY;
     static constexpr int kHaloItems = 6 * FX + 6 * FY;   // G halo values per plane
