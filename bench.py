@@ -33,6 +33,10 @@ Prints ONE JSON line (rank 0).  Options:
                            gradients (128^3, 8^3 S_L nodes + 16 mode amplitudes,
                            100 steps) for --sets seeded parameter sets vs
                            central finite differences on the forward solver
+  --config zalesak         the reference's own logged workloads instead: the 2D
+                           Zalesak rotation at 201^2..1601^2 (S_L 0, no reinit),
+                           end to end, against the ms/step of its RTX 4090 and
+                           32-rank MPI logs (vs_baseline: the 1601^2 4090 run)
 """
 import argparse
 import csv
@@ -289,6 +293,59 @@ def bench_adjoint(ls, mode, nsets, fd_sample=16, emit=True):
     return out
 
 
+# The reference's published run logs for the 2D Zalesak rotation (S_L = 0, no
+# reinitialisation, one full rotation): steps, final L2 error and ms per step
+# on an RTX 4090 (Local_code/level-set_GPU_2D/log/*.log) and on 32 MPI ranks
+# (Local_code/level-set_MPI_2D/log/*.log).
+ZALESAK_LOGS = {201: (6284, "8.576633e-02", 0.295, 0.389), 401: (12567, "8.373019e-02", 0.722, 1.263),
+                801: (25133, "8.250949e-02", 2.381, 4.631), 1601: (50266, "8.243448e-02", 9.510, 18.166)}
+
+
+def bench_zalesak(ls, mode):
+    """The reference's own 2D Zalesak workloads, timed end to end (one full
+    rotation, CFL 0.2, with L2 and area every 50 steps as the reference's GPU
+    loop does), against the ms/step its logs publish."""
+    import torch
+    T = 6.283185
+    rows = []
+    for N, (steps_ref, l2_ref, ms_4090, ms_mpi) in sorted(ZALESAK_LOGS.items()):
+        p = ls.default_params(2, N, reinit_interval=0, mode=ls.MODE_EXACT if mode == "exact" else ls.MODE_FAST)
+        dt = ls.cfl_dt(p, 0.2, 1.0, 0.0, 0.0)
+        G, u, v, _ = ls.init_case(p, ls.CASE_ZALESAK2D, period=T)
+        s = ls.Solver(p)
+        s.fill_ghosts(G)
+        G0 = G.clone()
+        s.advance(G.clone(), u, v, dt=dt, t_final=50 * dt, max_steps=50)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t, n = 0.0, 0
+        while n < steps_ref + 5:
+            k, t, _ = s.advance(G, u, v, dt=dt, t_final=T, step0=n, max_steps=50, t0=t)
+            n += k
+            l2, area = s.diag(G, G0)
+            if k < 50 or t >= T:
+                break
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = wall / n * 1e3
+        rows.append({"N": N, "steps": n, "log_steps": steps_ref, "ms_per_step": ms,
+                     "cell_updates_per_s": N * N / (ms * 1e-3), "final_l2": "%.6e" % l2, "log_l2": l2_ref,
+                     "reference_rtx4090_ms_per_step": ms_4090, "speedup_vs_rtx4090": ms_4090 / ms,
+                     "reference_mpi32_ms_per_step": ms_mpi, "speedup_vs_mpi32": ms_mpi / ms})
+        s.close()
+    big = rows[-1]
+    out = {"metric": "2D Zalesak rotation cell-updates/s (WENO5-RK3, S_L 0, no reinit, FP64) vs the reference's "
+                     "RTX 4090 log",
+           "value": big["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "steps": big["steps"],
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "vs_baseline": big["cell_updates_per_s"] / (1601 * 1601 / (9.510e-3)),
+           "config": {"workload": "Zalesak slotted disk, one rotation (T = 6.283185), CFL 0.2, periodic x / "
+                                  "zero-gradient y, L2 and area every 50 steps", "grid": [1601, 1601], "mode": mode},
+           "rows": rows}
+    print(json.dumps(out))
+    return out
+
+
 def bench_sample(ls, steps, n=800):
     """The sample/ scheme: the reference's Bunsen-flame driver loop (800 x 800,
     dt 2.5e-6, S_L 0.4, forced velocity at 10 Hz) on the GPU, with the
@@ -470,7 +527,8 @@ def main():
     ap.add_argument("--profile-pass", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--traffic-out", default=None)
-    ap.add_argument("--config", choices=["plane_modes", "batch", "adjoint", "sample"], default="plane_modes")
+    ap.add_argument("--config", choices=["plane_modes", "batch", "adjoint", "sample", "zalesak"],
+                    default="plane_modes")
     ap.add_argument("--sets", type=int, default=32)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
@@ -497,6 +555,9 @@ def main():
         return
     if a.config == "adjoint":
         bench_adjoint(ls, a.mode, a.sets)
+        return
+    if a.config == "zalesak":
+        bench_zalesak(ls, a.mode)
         return
     if a.config == "sample":
         bench_sample(ls, 2000 if a.steps == 20 else a.steps)
