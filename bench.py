@@ -36,7 +36,8 @@ Prints ONE JSON line (rank 0).  Options:
   --config zalesak         the reference's own logged workloads instead: the 2D
                            Zalesak rotation at 201^2..1601^2 (S_L 0, no reinit),
                            end to end, against the ms/step of its RTX 4090 and
-                           32-rank MPI logs (vs_baseline: the 1601^2 4090 run)
+                           32-rank MPI logs (vs_baseline: the 1601^2 4090 run),
+                           plus the 3D 64^3 deformation log workload
 """
 import argparse
 import csv
@@ -302,9 +303,9 @@ ZALESAK_LOGS = {201: (6284, "8.576633e-02", 0.295, 0.389), 401: (12567, "8.37301
 
 
 def bench_zalesak(ls, mode):
-    """The reference's own 2D Zalesak workloads, timed end to end (one full
-    rotation, CFL 0.2, with L2 and area every 50 steps as the reference's GPU
-    loop does), against the ms/step its logs publish."""
+    """The reference's own logged workloads, timed end to end against the
+    ms/step its logs publish: the 2D Zalesak rotation (one full rotation, CFL
+    0.2) and the 3D 64^3 deformation, with L2 and area every 50 steps."""
     import torch
     T = 6.283185
     rows = []
@@ -333,6 +334,31 @@ def bench_zalesak(ls, mode):
                      "reference_rtx4090_ms_per_step": ms_4090, "speedup_vs_rtx4090": ms_4090 / ms,
                      "reference_mpi32_ms_per_step": ms_mpi, "speedup_vs_mpi32": ms_mpi / ms})
         s.close()
+    # the 3D log workload: sphere in the time-reversed deformation field, 64^3,
+    # dt 0.001 to T = 1.5 (1501 steps, a sliver last step), velocity recomputed
+    # every step (Local_code/level-set_MPI_3D/log/simulation.log: 5.990 ms/step
+    # on 8 MPI ranks; level-set_GPU_3D: 10.653; level-set_MPI_GPU_3D: 14.973)
+    p = ls.default_params(3, 64, reinit_interval=0, velocity=ls.VEL_DEFORM3D, vel_period=1.5,
+                          mode=ls.MODE_EXACT if mode == "exact" else ls.MODE_FAST)
+    G, u, v, w = ls.init_case(p, ls.CASE_SPHERE_DEFORM3D, period=1.5)
+    s = ls.Solver(p)
+    s.fill_ghosts(G)
+    G0 = G.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, n = 0.0, 0
+    while True:
+        k, t, _ = s.advance(G, u, v, w, dt=0.001, t_final=1.5, step0=n, max_steps=50, t0=t)
+        n += k
+        l2, vol = s.diag(G, G0)
+        if k < 50 or t >= 1.5:
+            break
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / n * 1e3
+    deform = {"N": 64, "steps": n, "log_steps": 1501, "ms_per_step": ms, "final_l2": "%.6e" % l2,
+              "log_l2": "9.158634e-04", "reference_mpi8_ms_per_step": 5.990, "speedup_vs_mpi8": 5.990 / ms,
+              "reference_gpu_ms_per_step": 10.653, "speedup_vs_gpu": 10.653 / ms}
+    s.close()
     big = rows[-1]
     out = {"metric": "2D Zalesak rotation cell-updates/s (WENO5-RK3, S_L 0, no reinit, FP64) vs the reference's "
                      "RTX 4090 log",
@@ -341,7 +367,7 @@ def bench_zalesak(ls, mode):
            "vs_baseline": big["cell_updates_per_s"] / (1601 * 1601 / (9.510e-3)),
            "config": {"workload": "Zalesak slotted disk, one rotation (T = 6.283185), CFL 0.2, periodic x / "
                                   "zero-gradient y, L2 and area every 50 steps", "grid": [1601, 1601], "mode": mode},
-           "rows": rows}
+           "rows": rows, "deformation3d": deform}
     print(json.dumps(out))
     return out
 
