@@ -15,8 +15,11 @@ Prints ONE JSON line (rank 0).  Options:
   --no-adjoint             skip the reduced adjoint-vs-FD leg that the default
                            line carries under "adjoint" (4 parameter sets)
   --no-extra               skip the "other_mode" (the other arithmetic mode,
-                           10 steps of the same workload) and "batch" (the
-                           batched ensemble below) summaries of the default line
+                           10 steps of the same workload), "batch" (the
+                           batched ensemble below) and "reference_logs" (the
+                           1601^2 Zalesak and 64^3 deformation log workloads in
+                           exact mode, --config zalesak) summaries of the
+                           default line
   --profile-pass           launch a 1 GiB copy, a ghost fill and stage kernels
                            only (for rocprofv3 --pmc runs)
   --summarize-pmc F W      turn two rocprofv3 --pmc CSV directories (FETCH_SIZE,
@@ -302,7 +305,7 @@ ZALESAK_LOGS = {201: (6284, "8.576633e-02", 0.295, 0.389), 401: (12567, "8.37301
                 801: (25133, "8.250949e-02", 2.381, 4.631), 1601: (50266, "8.243448e-02", 9.510, 18.166)}
 
 
-def bench_zalesak(ls, mode):
+def bench_zalesak(ls, mode, sizes=None, emit=True):
     """The reference's own logged workloads, timed end to end against the
     ms/step its logs publish: the 2D Zalesak rotation (one full rotation, CFL
     0.2) and the 3D 64^3 deformation, with L2 and area every 50 steps."""
@@ -310,6 +313,8 @@ def bench_zalesak(ls, mode):
     T = 6.283185
     rows = []
     for N, (steps_ref, l2_ref, ms_4090, ms_mpi) in sorted(ZALESAK_LOGS.items()):
+        if sizes is not None and N not in sizes:
+            continue
         p = ls.default_params(2, N, reinit_interval=0, mode=ls.MODE_EXACT if mode == "exact" else ls.MODE_FAST)
         dt = ls.cfl_dt(p, 0.2, 1.0, 0.0, 0.0)
         G, u, v, _ = ls.init_case(p, ls.CASE_ZALESAK2D, period=T)
@@ -368,7 +373,8 @@ def bench_zalesak(ls, mode):
            "config": {"workload": "Zalesak slotted disk, one rotation (T = 6.283185), CFL 0.2, periodic x / "
                                   "zero-gradient y, L2 and area every 50 steps", "grid": [1601, 1601], "mode": mode},
            "rows": rows, "deformation3d": deform}
-    print(json.dumps(out))
+    if emit:
+        print(json.dumps(out))
     return out
 
 
@@ -621,6 +627,15 @@ def main():
                                            "speedup_vs_serial", "bitwise_equal_to_serial")}
         out["batch"]["workload"] = bt["config"]["workload"]
         out["batch"]["tile_shape"] = bt["config"]["tile_shape"]
+        # the reference's own logged workloads (the only published timings), exact mode
+        rl = bench_zalesak(ls, "exact", sizes=(1601,), emit=False)
+        z, dfm = rl["rows"][-1], rl["deformation3d"]
+        out["reference_logs"] = {
+            "zalesak2d_1601": {k: z[k] for k in ("steps", "ms_per_step", "final_l2", "log_l2",
+                                                 "reference_rtx4090_ms_per_step", "speedup_vs_rtx4090")},
+            "deformation3d_64": {k: dfm[k] for k in ("steps", "ms_per_step", "final_l2", "log_l2",
+                                                     "reference_gpu_ms_per_step", "speedup_vs_gpu")},
+            "mode": "exact"}
     print(json.dumps(out), flush=True)
 
 
