@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 
 #include "ls_api.h"
 #include "ls_exact.hpp"
@@ -248,7 +249,17 @@ struct ls_solver {
     DiagPart* diag;     // kRedBlocks + 1 partials (device)
     DiagPart* diag_host;  // pinned host copy of the final partial
     int device;
+    // ls_advance's plain RK3 loop as a captured graph of kGraphSteps steps
+    // (made on first use, kept for the arguments in gkey/gdt)
+    hipStream_t cap = nullptr;        // private stream the graph is captured on
+    hipGraphExec_t gexec = nullptr;
+    const void* gkey[6] = {};         // cur, a, u, v, w, sl at capture
+    double gdt = 0.0;
 };
+
+// Steps per graph replay.  Even, so that the buffer rotation of the loop is
+// back where it started after each replay.
+constexpr int kGraphSteps = 16;
 
 namespace lsi {
 
@@ -476,6 +487,8 @@ int ls_create(const ls_params_t* p, ls_solver_t** out) {
 
 int ls_destroy(ls_solver_t* s) {
     if (!s) return LS_OK;
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->cap) (void)hipStreamDestroy(s->cap);
     (void)hipFree(s->buf[0]);
     (void)hipFree(s->buf[1]);
     (void)hipFree(s->diag);
@@ -546,6 +559,37 @@ int ls_reinit_hcr2(ls_solver_t* s, double* G, void* stream) {
     return LS_OK;
 }
 
+// Captures kGraphSteps RK3 steps of the plain loop (state in cur, spare
+// buffers a and b) into s->gexec, on the solver's private stream.  Nothing runs.
+static int capture_loop(ls_solver* s, double* cur, double* a, double* b, const double* u,
+                        const double* v, const double* w, const double* sl, double dt) {
+    if (s->gexec) {
+        (void)hipGraphExecDestroy(s->gexec);
+        s->gexec = nullptr;
+    }
+    if (!s->cap) LS_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+    LS_HIP(hipStreamBeginCapture(s->cap, hipStreamCaptureModeThreadLocal));
+    int rc = LS_OK;
+    for (int q = 0; q < kGraphSteps && rc == LS_OK; q++) {
+        rc = rk3(s, cur, a, a, b, u, v, w, sl, dt, s->cap);
+        std::swap(cur, a);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s->cap, &graph);
+    if (rc != LS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(LS_ERR_DEVICE, "graph capture of the RK3 loop failed");
+    const hipError_t ei = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        s->gexec = nullptr;
+        return fail(LS_ERR_DEVICE, "graph instantiation of the RK3 loop failed");
+    }
+    return LS_OK;
+}
+
 int ls_advance(ls_solver_t* s, double* G, double* u, double* v, double* w,
                const double* sl_field, double dt, double t_final, int64_t step0,
                int64_t max_steps, double* t_io, int64_t* steps_done, double* dt_last,
@@ -564,6 +608,30 @@ int ls_advance(ls_solver_t* s, double* G, double* u, double* v, double* w,
     double* a = s->buf[0];
     double* b = s->buf[1];
     int rc;
+    // Without reinitialisation and with fixed velocity arrays, runs of
+    // kGraphSteps full steps replay a captured graph: the same launches with
+    // the same arguments, so the same bits, at a fraction of the launch cost
+    // (the small grids are launch-bound).  The host still advances t step by
+    // step, so the loop's time and sliver step are unchanged.
+    const bool plain = ri == 0 && s->p.velocity != LS_VEL_DEFORM3D;
+    while (plain && step + kGraphSteps <= stop) {
+        double tt = t;
+        bool full = true;
+        for (int q = 0; q < kGraphSteps && full; q++) {
+            if (!(tt < t_final) || tt + dt > t_final) full = false;
+            tt += dt;
+        }
+        if (!full) break;
+        const void* key[6] = {cur, a, u, v, w, sl_field};
+        if (!s->gexec || s->gdt != dt || std::memcmp(key, s->gkey, sizeof(key)) != 0) {
+            if ((rc = capture_loop(s, cur, a, b, u, v, w, sl_field, dt))) return rc;
+            std::memcpy(s->gkey, key, sizeof(key));
+            s->gdt = dt;
+        }
+        LS_HIP(hipGraphLaunch(s->gexec, st));
+        t = tt;
+        step += kGraphSteps;
+    }
     while (t < t_final && step < stop) {
         if (t + dt > t_final) dt = t_final - t;
         if (s->p.velocity == LS_VEL_DEFORM3D) {

@@ -269,3 +269,42 @@ def test_rk3_step_extreme_magnitudes_bitwise(ls, oracle, cuda, dim, n, bc, scale
     got = out.cpu().numpy()
     assert np.isfinite(ref).all()
     assert np.array_equal(got, ref), "rel Linf %.3e" % rel_linf(got, ref)
+
+
+@pytest.mark.parametrize("dim,n,bc", [CASES[0], CASES[5], CASES[8]])
+def test_advance_graph_replays_bitwise(ls, oracle, cuda, dim, n, bc):
+    """ls_advance without reinitialisation replays a captured graph of 16 RK3
+    steps.  Three calls in a row (the second reuses the graph, the third has
+    a new dt and recaptures), with eager stage launches and a synchronisation
+    between them, each bitwise equal to the same steps as eager rk3_step
+    launches; the first also to the oracle's loop, sliver step included."""
+    print("advance graph", dim, n, bc, flush=True)
+    g, p = _grid_and_params(ls, dim, n, bc, 0.1, ri=0)
+    G, vel = _inputs(g, seed=37, vel_scale=0.5)
+    G = oracle.fill_ghosts(g, G)
+    w = vel[2] if dim == 3 else None
+    s = ls.Solver(p)
+    dG = _dev(cuda, G)
+    du, dv, dw = _dev(cuda, vel[0]), _dev(cuda, vel[1]), _dev(cuda, w)
+    eager = dG.clone()
+    tmp = s.empty()
+    t, step = 0.0, 0
+    h = min(g.dx, g.dy)
+    for call, (dt, nsteps) in enumerate([(0.3 * h, 37), (0.3 * h, 21), (0.2 * h, 40)]):
+        t_final = t + (nsteps - 0.63) * dt  # the last step is a sliver
+        k, t_gpu, dl = s.advance(dG, du, dv, dw, dt=dt, t_final=t_final, step0=step, t0=t)
+        assert k == nsteps and dl < dt
+        for q in range(k):  # the same steps as eager launches, the sliver last
+            s.rk3_step(eager, tmp, du, dv, dw, dt=dt if q < k - 1 else dl)
+            eager, tmp = tmp, eager
+        cuda.cuda.synchronize()
+        got = dG.cpu().numpy()
+        assert np.isfinite(got).all()
+        assert np.array_equal(got, eager.cpu().numpy()), "call %d: graph vs eager" % call
+        if call == 0:
+            ref, k_ref, t_ref, dl_ref = oracle.advance(g, G, vel[0], vel[1], w, dt=dt, t_final=t_final,
+                                                       reinit_interval=0)
+            assert (k, t_gpu, dl) == (k_ref, t_ref, dl_ref)
+            assert np.array_equal(got, ref), "vs oracle"
+        t, step = t_gpu, step + k
+    s.close()
