@@ -171,7 +171,10 @@ int ls_hcr2_crossings(ls_solver_t* s, const double* phi_0, double* r_tilde, int3
  * step counts from step0 (the global step index of the first step).  On
  * return *t_io holds the final time, *steps_done the number of steps and
  * *dt_last the last dt used (any of them may be NULL except t_io).
- * Synchronises the stream before returning. */
+ * With ri == 0 and velocity arrays, runs of full steps replay a hipGraph that
+ * the solver captures on a private stream on first use and keeps (for the
+ * same pointers and dt) until ls_destroy; the results are those of the
+ * launches themselves.  Synchronises the stream before returning. */
 int ls_advance(ls_solver_t* s, double* G, double* u, double* v, double* w,
                const double* sl_field, double dt, double t_final, int64_t step0,
                int64_t max_steps, double* t_io, int64_t* steps_done, double* dt_last,
