@@ -152,29 +152,8 @@ def cpu_baseline(n):
 
 
 def ensemble(ls, count=96, seed=4, mode="fast"):
-    """The batched workload: seeded member sizes, flame speeds and inputs."""
-    import numpy as np
-    import torch
-    rng = np.random.default_rng(seed)
-    sizes = [128, 192, 256, 384, 512]
-    members, params = [], []
-    for q in range(count):
-        n = int(sizes[rng.integers(0, len(sizes))])
-        s_l = float(rng.uniform(0.05, 0.2))
-        p = ls.default_params(2, n, s_l=s_l, bc=(ls.BC_ZERO_GRAD, ls.BC_ZERO_GRAD, 0), reinit_interval=0,
-                              mode=ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT)
-        params.append(p)
-        members.append(dict(nx=n, ny=n, bc=(ls.BC_ZERO_GRAD, ls.BC_ZERO_GRAD), dx=p.dx, dy=p.dy, s_l=s_l,
-                            dt=0.0))
-    b0 = None
-    fields = []
-    for q, p in enumerate(params):
-        G, u, v, _ = ls.init_case(p, ls.CASE_FLAME2D, seed=1000 * seed + q)
-        speed = (u.abs() + v.abs()).max().item()
-        members[q]["dt"] = 0.5 * min(p.dx, p.dy) / (speed + p.s_l)
-        fields.append((G, u, v))
-    torch.cuda.synchronize()
-    return members, params, fields
+    """The batched workload (the package's ensemble builder)."""
+    return ls.ensemble(count, seed, ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT)
 
 
 def bench_batch(ls, steps, mode, emit=True):

@@ -367,6 +367,36 @@ class Batch:
         return self.tile(), ms.value
 
 
+ENSEMBLE_SIZES = (128, 192, 256, 384, 512)
+
+
+def ensemble(count=96, seed=4, mode=MODE_FAST):
+    """The batched-ensemble workload: ``count`` 2D flames with seeded edge
+    lengths from ENSEMBLE_SIZES, S_L ~ U[0.05, 0.2], zero-gradient edges and
+    the seeded circle/plane G0 and uniform + 8-mode velocity of
+    LS_CASE_FLAME2D (member q uses seed 1000*seed + q); dt from CFL 0.5 on
+    the member's largest |u| + |v|.  Returns (members, params, fields) with
+    fields[q] = (G, u, v) padded device tensors as generated (ghost cells
+    hold the analytic fields; fill G's ghosts before stepping)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    members, params, fields = [], [], []
+    for q in range(count):
+        n = int(ENSEMBLE_SIZES[rng.integers(0, len(ENSEMBLE_SIZES))])
+        s_l = float(rng.uniform(0.05, 0.2))
+        p = default_params(2, n, s_l=s_l, bc=(BC_ZERO_GRAD, BC_ZERO_GRAD, 0), reinit_interval=0, mode=mode)
+        params.append(p)
+        members.append(dict(nx=n, ny=n, bc=(BC_ZERO_GRAD, BC_ZERO_GRAD), dx=p.dx, dy=p.dy, s_l=s_l, dt=0.0))
+    for q, p in enumerate(params):
+        G, u, v, _ = init_case(p, CASE_FLAME2D, seed=1000 * seed + q)
+        speed = (u.abs() + v.abs()).max().item()
+        members[q]["dt"] = 0.5 * min(p.dx, p.dy) / (speed + p.s_l)
+        fields.append((G, u, v))
+    torch.cuda.synchronize()
+    return members, params, fields
+
+
 # ---------------------------------------------------------------------------
 # Discrete adjoint (3D)
 

@@ -8,6 +8,7 @@
 // usage: ls_solve [-dim 2|3] [-n N] [-t zalesak|circle|sphere|swirl|plane]
 //                 [-T t] [-cfl c] [-sl s] [-u u] [-v v] [-w w] [-reinit|-no-reinit]
 //                 [-ri N] [-riter N] [-mode exact|fast] [-o dir] [-no-output] [-q]
+//                 [-dump-crossings]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -29,14 +30,16 @@ static void usage(const char* prog) {
                 "  -reinit / -no-reinit, -ri <N>, -riter <N>   HCR-2 settings\n"
                 "  -mode <exact|fast>  arithmetic mode (default exact)\n"
                 "  -o <dir>       output directory (default ./output)\n"
-                "  -no-output     no G files\n  -q             quiet\n", prog);
+                "  -no-output     no G files\n  -q             quiet\n"
+                "  -dump-crossings  write phi_0, r_tilde and the crossing flags of the last\n"
+                "                 reinitialisation to <dir>/{phi_0,r_tilde,flag}.raw (padded, no header)\n", prog);
 }
 
 int main(int argc, char** argv) {
     int dim = 2, n = -1, mode = LS_MODE_EXACT, ri = 10, riter = 2;
     std::string test, outdir = "./output";
     double T = -1.0, cfl = -1.0, sl = 0.0, u = 0.0, v = 0.0, w = 0.0;
-    bool have_u = false, reinit = false, save = true, verbose = true;
+    bool have_u = false, reinit = false, save = true, verbose = true, dump = false;
     for (int a = 1; a < argc; a++) {
         auto next = [&](const char* f) -> const char* {
             if (a + 1 >= argc) { std::fprintf(stderr, "missing value for %s\n", f); std::exit(1); }
@@ -60,6 +63,7 @@ int main(int argc, char** argv) {
         else if (!std::strcmp(argv[a], "-o")) outdir = next("-o");
         else if (!std::strcmp(argv[a], "-no-output")) save = false;
         else if (!std::strcmp(argv[a], "-q")) verbose = false;
+        else if (!std::strcmp(argv[a], "-dump-crossings")) dump = true;
         else { std::fprintf(stderr, "Unknown option: %s\n", argv[a]); usage(argv[0]); return 1; }
     }
     if (test.empty()) test = dim == 3 ? "sphere" : "zalesak";
@@ -98,7 +102,8 @@ int main(int argc, char** argv) {
         }
     }
     int* flag = nullptr;
-    if (hipMalloc((void**)&flag, N * sizeof(int)) != hipSuccess) return 1;
+    if (hipMalloc((void**)&flag, N * sizeof(int)) != hipSuccess || hipMemset(flag, 0, N * sizeof(int)) != hipSuccess)
+        return 1;
     double *G = f[0], *G_new = f[1], *G_1 = f[2], *G_2 = f[3], *G_init = f[4], *du = f[5], *dv = f[6],
            *dw = f[7], *tmp = f[8], *p0 = f[9], *rt = f[10];
     lsref::check(ls_init_case(&p, case_id, 3, sp.t_final, G, du, dv, dim == 3 ? dw : nullptr, nullptr), "init");
@@ -118,10 +123,10 @@ int main(int argc, char** argv) {
     std::printf("  Grid size: %d^%d, dx: %.6e, dt: %.6e, T: %.4f, S_L: %.4f, reinit: %s\n", n, dim, sp.dx,
                 sp.dt, sp.t_final, sp.s_l, reinit ? "Enabled" : "Disabled");
     std::printf("Initial interface %s: %.6f\n", dim == 3 ? "volume" : "area", a0);
-    if (save) {
+    if (save || dump) {
         std::string cmd = "mkdir -p " + outdir;
         if (std::system(cmd.c_str()) != 0) std::fprintf(stderr, "could not create %s\n", outdir.c_str());
-        saveFieldBinary((outdir + "/G_initial.bin").c_str(), G, sp);
+        if (save) saveFieldBinary((outdir + "/G_initial.bin").c_str(), G, sp);
     }
 
     const int output_interval = dim == 3 ? 50 : 10;
@@ -161,6 +166,22 @@ int main(int argc, char** argv) {
         saveFieldBinary((outdir + "/G_final.bin").c_str(), G, sp);
         // the 3D driver also writes a VTK copy (Local_code/level-set_MPI_3D/src/main.cpp:377-380)
         if (dim == 3) saveFieldVTK3D((outdir + "/G_final.vtk").c_str(), G, sp);
+    }
+    if (dump) {
+        // the arrays the shim's computeInterfaceCrossings filled at the last
+        // reinitializeWithSwap, as the host sees them
+        auto raw = [&](const char* name, const void* dev, size_t sz) {
+            std::vector<char> h(sz);
+            if (hipMemcpy(h.data(), dev, sz, hipMemcpyDeviceToHost) != hipSuccess) return false;
+            FILE* fp = std::fopen((outdir + "/" + name).c_str(), "wb");
+            if (!fp) return false;
+            const bool ok = std::fwrite(h.data(), 1, sz, fp) == sz;
+            return std::fclose(fp) == 0 && ok;
+        };
+        if (!raw("phi_0.raw", p0, bytes) || !raw("r_tilde.raw", rt, bytes) || !raw("flag.raw", flag, N * sizeof(int))) {
+            std::fprintf(stderr, "could not write the crossing arrays to %s\n", outdir.c_str());
+            return 1;
+        }
     }
     for (auto ptr : f) hipFree(ptr);
     hipFree(flag);

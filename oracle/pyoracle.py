@@ -238,6 +238,14 @@ class Reference(_Lib):
         assert rc == 0
         return G
 
+    def crossings(self, g, p0):
+        """computeInterfaceCrossings[3D]: (r_tilde, flag); ghost entries stay 0."""
+        rt = np.zeros_like(p0)
+        fl = np.zeros(p0.shape, np.int32)
+        rc = self.lib.ref_crossings(C.byref(g), _p(np.ascontiguousarray(p0)), _p(rt), _pi(fl))
+        assert rc == 0
+        return rt, fl
+
     def advance(self, g, G, u, v, w=None, dt=1e-3, t_final=1.0, max_steps=None,
                 reinit_interval=0, deform_T=0.0):
         G = G.copy()

@@ -213,6 +213,19 @@ int ref_hcr2(const ref_grid* g, double* G, double* work, int* flag, int direct) 
     return 0;
 }
 
+// computeInterfaceCrossings[3D] on phi_0 as given (its ghosts already
+// filled by the caller): r~ and the crossing bit mask on interior cells.
+int ref_crossings(const ref_grid* g, const double* p0, double* rt, int* flag) {
+    ensure_mpi();
+    SimParams p = mk(g);
+#if REF_DIM == 2
+    computeInterfaceCrossings(p0, rt, flag, p);
+#else
+    computeInterfaceCrossings3D(p0, rt, flag, p);
+#endif
+    return 0;
+}
+
 // Time loop with the semantics of main.cpp:310-354 (3D :287-334).
 // deform_T > 0: recompute the 3D deformation velocity with the reference's
 // updateDeformationVelocity at the start of each step.  work: 7 padded

@@ -119,32 +119,50 @@ def test_fast_3d_workloads_reduced(ls, oracle, cuda, case, n, steps):
     front_parity(got, ref, _h(p), TOL, g.interior)
 
 
-@pytest.mark.parametrize("case,n,steps", [("sphere_swirl", 256, 40), ("plane_modes", 512, 1), ("plane_modes", 512, 100)])
-def test_fast_full_size_vs_exact(ls, cuda, case, n, steps):
+@pytest.mark.parametrize("case,n,steps", [("sphere_swirl", 256, 60), ("plane_modes", 512, 100)])
+def test_fast_full_size_every_step(ls, cuda, case, n, steps):
     """The full-size 3D workloads (sphere in swirl 256^3, plane with modes 512^3):
-    fast mode against the exact mode, one
-    step and over the benchmark's 100-step horizon (HCR-2 every 5 steps).
-    The gap is not monotonic in time: discrete events near zero (an upwind
-    choice at u_eff ~ 0, an HCR-2 stability test) can open a local ~1e-12
-    difference that the scheme damps again; measured at 512^3 it is 4e-16
-    after 1 step, ~8e-13 after 20 and ~2e-14 after 100."""
+    fast mode against the exact mode (bitwise the oracle; the oracle itself is
+    compared at these sizes in test_gpu_workloads.py) after EVERY step of the
+    horizon, HCR-2 after every 5th step, steps counted from 1 as the benchmark
+    counts them (its timed region covers steps 4..23 or 6..25).  Both 1e-12
+    bars hold at every step: relative L-infinity in G and the interpolated
+    front position on every grid edge.  The gap is not monotonic in time:
+    discrete events near zero (an upwind choice at u_eff ~ 0, an HCR-2
+    stability test) open local differences that the scheme damps again; the
+    largest over the horizon is printed."""
+    from helpers import front_parity_torch
     s_l = 0.05 if case == "sphere_swirl" else 0.08
     cid = ls.CASE_SPHERE_SWIRL3D if case == "sphere_swirl" else ls.CASE_PLANE_MODES3D
-    res = []
+    run = []
     for mode in (ls.MODE_EXACT, ls.MODE_FAST):
         p = ls.default_params(3, n, s_l=s_l, reinit_interval=5, mode=mode)
         G, u, v, w = ls.init_case(p, cid, seed=3)
         s = ls.Solver(p)
         s.fill_ghosts(G)
-        speed = (u.abs() + v.abs() + w.abs()).max().item()
-        dt = 0.5 * p.dx / (speed + s_l)
-        s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=1, max_steps=steps)
-        res.append(G.cpu().numpy())
-        del G, u, v, w, s
-        cuda.cuda.empty_cache()
+        run.append((s, G, u, v, w))
+    speed = (run[0][2].abs() + run[0][3].abs() + run[0][4].abs()).max().item()
+    dt = 0.5 * p.dx / (speed + s_l)
+    h = (p.dz, p.dy, p.dx)
     I = (slice(3, 3 + n),) * 3
-    assert rel_linf(res[1], res[0], I) <= TOL
-    front_parity(res[1], res[0], (p.dz, p.dy, p.dx), TOL, I)
+    worst_g, worst_f, at = 0.0, 0.0, 0
+    for k in range(1, steps + 1):
+        for s, G, u, v, w in run:
+            done, _, _ = s.advance(G, u, v, w, dt=dt, t_final=1e30, step0=k, max_steps=1)
+            assert done == 1
+        ge, gf = run[0][1][I], run[1][1][I]
+        err = float((gf - ge).abs().max()) / float(ge.abs().max())
+        fr, _ = front_parity_torch(gf, ge, h, TOL)
+        if err > worst_g:
+            worst_g, at = err, k
+        worst_f = max(worst_f, fr)
+        assert err <= TOL, "step %d: fast vs exact %.3e" % (k, err)
+    print("fast vs exact %s %d^3, steps 1..%d: max rel L-inf in G %.3e (step %d), max front offset %.3e L"
+          % (case, n, steps, worst_g, at, worst_f))
+    for s, *_ in run:
+        s.close()
+    del run
+    cuda.cuda.empty_cache()
 
 
 def test_fast_log_known_answers(ls, cuda):

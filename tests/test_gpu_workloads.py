@@ -15,8 +15,12 @@
    the 3D sphere-in-swirl and plane-with-modes workloads at reduced size:
    bitwise against the oracle.
 3. The sphere-in-swirl (256^3) and plane-with-modes (512^3) workloads at full
-   size, where the CPU oracle would take hours: periodic translation
-   equivariance (bitwise) and run-to-run determinism.
+   size against the oracle's time loop (Local_code/level-set_MPI_3D/src/main.cpp:285-334):
+   21 steps at 256^3 (HCR-2 after steps 5, 10, 15, 20) and 6 steps at 512^3
+   (HCR-2 after step 5).  The OpenMP oracle takes about 2 s per 512^3 step on
+   16 host cores.  Exact mode bitwise, fast mode within 1e-12 in G and front
+   position.  Also periodic translation equivariance (bitwise) and run-to-run
+   determinism over 6 steps.
 """
 import math
 
@@ -167,3 +171,41 @@ def test_full_size_translation_equivariance(ls, cuda, case, n, steps, s_l):
     s.fill_ghosts(G2)
     s.advance(G2, u2, v2, w2, dt=dt, t_final=1e30, step0=1, max_steps=steps)
     assert cuda.equal(G, G2)
+
+
+@pytest.mark.parametrize("case,n,steps", [("sphere_swirl", 256, 21), ("plane_modes", 512, 6)])
+def test_full_size_vs_oracle(ls, oracle, cuda, case, n, steps):
+    """The full-size workloads against the oracle's loop on the same seeded
+    inputs (steps counted from 0: HCR-2 after every 5th step)."""
+    import time
+    from helpers import front_parity, rel_linf
+    s_l = 0.05 if case == "sphere_swirl" else 0.08
+    cid = ls.CASE_SPHERE_SWIRL3D if case == "sphere_swirl" else ls.CASE_PLANE_MODES3D
+    got = {}
+    for mode in (ls.MODE_EXACT, ls.MODE_FAST):
+        p = ls.default_params(3, n, s_l=s_l, reinit_interval=5, mode=mode)
+        G, u, v, w = ls.init_case(p, cid, seed=3)
+        s = ls.Solver(p)
+        s.fill_ghosts(G)
+        speed = (u.abs() + v.abs() + w.abs()).max().item()
+        dt = 0.5 * p.dx / (speed + s_l)
+        if mode == ls.MODE_EXACT:
+            host = [a.cpu().numpy() for a in (G, u, v, w)]
+        done, _, _ = s.advance(G, u, v, w, dt=dt, t_final=1e30, max_steps=steps)
+        assert done == steps
+        got[mode] = G.cpu().numpy()
+        s.close()
+        del G, u, v, w
+        cuda.cuda.empty_cache()
+    g = make_grid(3, n, n, n, bc=(0, 0, 0), s_l=s_l)
+    t0 = time.perf_counter()
+    ref, k, _, _ = oracle.advance(g, *host, dt=dt, t_final=1e30, max_steps=steps, reinit_interval=5)
+    print("oracle %s %d^3: %d steps in %.1f s" % (case, n, k, time.perf_counter() - t0))
+    del host
+    assert k == steps
+    assert np.array_equal(got[ls.MODE_EXACT], ref), "exact mode differs from the oracle"
+    I = g.interior
+    err = rel_linf(got[ls.MODE_FAST], ref, I)
+    front = front_parity(got[ls.MODE_FAST], ref, (g.dz, g.dy, g.dx), 1e-12, I)
+    print("fast vs oracle %s %d^3 after %d steps: rel L-inf %.3e, front %.3e L" % (case, n, steps, err, front))
+    assert err <= 1e-12

@@ -303,3 +303,57 @@ def test_write_G_format(oracle, tmp_path):
         q = tmp_path / "r3.bin"
         Reference(3).write_G(g3, G3, q)
         assert q.read_bytes() == raw
+
+
+CROSSINGS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "crossings_vectors.npz")
+
+
+def _crossing_cases():
+    z = np.load(CROSSINGS)
+    return sorted({k.split("/")[0] for k in z.files})
+
+
+@pytest.mark.parametrize("name", _crossing_cases())
+def test_golden_crossings(oracle, name):
+    """computeInterfaceCrossings[3D] (Local_code/level-set_MPI_2D/include/reinitialization.h:75-130,
+    3D :46-89): the oracle's r~ and bit mask equal the reference's bit for bit,
+    including the planted cells of make_crossings_golden.py."""
+    z = np.load(CROSSINGS)
+    m = z[name + "/meta"]
+    dim = int(m[0])
+    g = make_grid(dim, int(m[1]), int(m[2]), int(m[3]), bc=(int(m[4]), int(m[5]), int(m[6])))
+    p0 = z[name + "/p0"]
+    rt, fl = oracle.crossings(g, p0)
+    assert np.array_equal(rt, z[name + "/rt"]) and np.array_equal(fl, z[name + "/flag"])
+    # the planted centre cell: two crossing neighbours summing to -9e-16, below
+    # the 1e-15 gate, so r~ = 0 while both bits are set
+    c = tuple(s // 2 for s in g.shape)
+    lo = [1 << (2 * (dim - 1 - a)) for a in range(dim)]   # the low-side bit of array axis a
+    hi = [b << 1 for b in lo]
+    assert rt[c] == 0.0 and fl[c] == lo[0] | hi[1]
+    # 3 cells to the low x side: the sum -1.1e-15 passes the gate
+    c2 = c[:-1] + (c[-1] - 3,)
+    assert fl[c2] == lo[0] | hi[1] and rt[c2] == 0.3 / (-6e-16 + -5e-16)
+    # a centre of exactly 0 has no crossing (the product is 0, not < 0)
+    c3 = c[:-1] + (c[-1] + 3,)
+    assert fl[c3] == 0 and rt[c3] == 0.0
+    # every ghost entry is left at 0
+    mask = np.zeros(g.shape, bool)
+    mask[g.interior] = True
+    assert not fl[~mask].any() and not rt[~mask].any()
+    # flagged neighbours have the opposite sign of phi_c, so r~ <= 0 everywhere
+    assert (rt <= 0.0).all() and (rt < 0.0).any()
+
+
+@pytest.mark.skipif(not (ref_available(2) and ref_available(3)), reason="reference build absent")
+@pytest.mark.parametrize("dim", [2, 3])
+def test_live_reference_crossings(oracle, dim):
+    R = Reference(dim)
+    n = (19, 14, 12)
+    g = make_grid(dim, *n, bc=(0, 0, 0) if dim == 3 else (0, 1, 0))
+    h = (g.dz, g.dy, g.dx) if dim == 3 else (g.dy, g.dx)
+    for seed in range(3):
+        p0 = R.fill_ghosts(g, bumpy_sdf(g.shape, h, 50 + seed, noise=0.05))
+        a, b = oracle.crossings(g, p0), R.crossings(g, p0)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert (a[1] != 0).sum() > 10
