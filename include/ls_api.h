@@ -190,6 +190,22 @@ int ls_diag(ls_solver_t* s, const double* G, const double* G0, double* l2, doubl
  * Synchronises the stream. */
 int ls_check_finite(ls_solver_t* s, const double* G, void* stream);
 
+/* ls_diag without a solver handle and without allocating.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and summation order as ls_diag; the results
+ * go to DEVICE memory, stream-ordered, with no synchronisation:
+ *   out[0] = sqrt(mean((G-G0)^2)) over interior cells (0 when G0 is NULL),
+ *   out[1] = count(G < 0) * dx*dy[*dz],
+ *   out[2] = number of non-finite interior values of G,
+ * bitwise the values ls_diag and ls_check_finite see.  The partial sums live
+ * in a static device array of fixed size (1025 partials, 24,600 bytes; the
+ * reduction runs 1024 blocks that stride over the interior, so its size does
+ * not grow with the grid and every grid ls_create accepts fits), one per GPU,
+ * kept until the process ends.  Calls that overlap on one GPU (two streams or
+ * two host threads) race on it.  Rejects the parameters ls_create rejects. */
+int ls_diag_noalloc(const ls_params_t* p, const double* G, const double* G0, double* out, void* stream);
+
 /* Write a HOST copy of a padded field in the reference's binary G format:
  * int32 {nx, ny, nghost} (2D) or {nx, ny, nz, nghost} (3D), then the padded
  * doubles with the y-ghost rows replaced by the adjacent interior row. */

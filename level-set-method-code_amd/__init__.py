@@ -103,6 +103,7 @@ def lib():
                              C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double), vp]
     L.ls_diag.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
     L.ls_check_finite.argtypes = [vp, vp, vp]
+    L.ls_diag_noalloc.argtypes = [P, vp, vp, vp, vp]
     L.ls_write_G.argtypes = [P, vp, C.c_char_p]
     L.ls_write_G_vtk.argtypes = [P, vp, C.c_char_p]
     L.ls_init_case.argtypes = [P, C.c_int32, C.c_uint64, C.c_double, vp, vp, vp, vp, vp]
@@ -281,6 +282,23 @@ class Solver:
     def check_finite(self, G, stream=None):
         self._chk(G)
         return lib().ls_check_finite(self.h, _ptr(G), _stream(stream))
+
+
+def diag_noalloc(p, G, G0=None, out=None, stream=None):
+    """ls_diag_noalloc: (l2, area, non-finite count) of G as a 3-element
+    float64 device tensor, stream-ordered, no solver handle, no allocation in
+    the library (``out`` may be preallocated by the caller)."""
+    import torch
+    if out is None:
+        out = torch.empty(3, dtype=torch.float64, device="cuda")
+    for t in (G, G0):
+        if t is not None and (not t.is_cuda or t.dtype != torch.float64 or tuple(t.shape) != tuple(p.shape)
+                              or not t.is_contiguous()):
+            raise ValueError("fields must be contiguous float64 device tensors of shape %s" % (p.shape,))
+    if not out.is_cuda or out.dtype != torch.float64 or out.numel() < 3 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float64 device tensor of 3 elements")
+    check(lib().ls_diag_noalloc(C.byref(p), _ptr(G), _ptr(G0), _ptr(out), _stream(stream)))
+    return out
 
 
 def init_case(p, case_id, seed=0, period=1.0, stream=None):

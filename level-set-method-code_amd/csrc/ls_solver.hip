@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 
@@ -235,6 +236,49 @@ __global__ __launch_bounds__(256) void k_diag_final(DiagPart* __restrict__ part,
         __syncthreads();
     }
     if (threadIdx.x == 0) part[nparts] = DiagPart{s_sq[0], s_neg[0], s_bad[0]};
+}
+
+// The final partial as the three numbers ls_diag reports, on the device:
+// the same correctly rounded quotient, square root and product as the host
+// side of ls_diag, so the values are the same bits.
+__global__ __launch_bounds__(64) void k_diag_emit(const DiagPart* __restrict__ fin, double* __restrict__ out,
+                                                  double ncell, double cell, int have_g0) {
+    if (threadIdx.x == 0) {
+        const DiagPart r = *fin;
+        out[0] = have_g0 ? sqrt(r.sq / ncell) : 0.0;
+        out[1] = (double)r.neg * cell;
+        out[2] = (double)r.bad;
+    }
+}
+
+// Partial sums of ls_diag_noalloc: kRedBlocks block partials and the final
+// one (24,600 bytes), one copy per GPU, held until the process ends.  Every
+// call overwrites all of it before reading it, so nothing needs clearing;
+// calls that overlap on one GPU (two streams, or two host threads) race on it.
+// Host code never uses the symbol itself, only the address that
+// hipGetSymbolAddress gives for the current device.
+static __device__ __align__(256) DiagPart scratch[kRedBlocks + 1];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+DiagPart* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call for that device and kept.
+int scratch_on_device(DiagPart** out) {
+    int dev = 0;
+    LS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(LS_ERR_INVALID, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(LS_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<DiagPart*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return LS_OK;
 }
 
 }  // namespace
@@ -688,6 +732,25 @@ int ls_check_finite(ls_solver_t* s, const double* G, void* stream) {
     LS_HIP(hipMemcpyAsync(s->diag_host, s->diag + kRedBlocks, sizeof(DiagPart), hipMemcpyDeviceToHost, st));
     LS_HIP(hipStreamSynchronize(st));
     if (s->diag_host->bad != 0) return fail(LS_ERR_NONFINITE, "non-finite value in G");
+    return LS_OK;
+}
+
+int ls_diag_noalloc(const ls_params_t* p, const double* G, const double* G0, double* out, void* stream) {
+    int rc = lsi::validate(p);
+    if (rc) return rc;
+    if (!G || !out) return fail(LS_ERR_INVALID, "NULL argument");
+    DiagPart* part = nullptr;
+    rc = scratch_on_device(&part);
+    if (rc) return rc;
+    hipStream_t st = S(stream);
+    const Geom g = make_geom(*p);
+    const int64_t ncell = (int64_t)g.nx * g.ny * g.nz;
+    const double cell = g.dim == 3 ? g.dx * g.dy * g.dz : g.dx * g.dy;
+    hipLaunchKernelGGL(k_diag_partial, dim3(kRedBlocks), dim3(256), 0, st, g, G, G0, part, ncell);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, part, kRedBlocks);
+    hipLaunchKernelGGL(k_diag_emit, dim3(1), dim3(64), 0, st, (const DiagPart*)(part + kRedBlocks), out,
+                       (double)ncell, cell, G0 ? 1 : 0);
+    LS_HIP(hipGetLastError());
     return LS_OK;
 }
 
