@@ -677,16 +677,17 @@ int ls_adjoint_loss(ls_solver_t* s, const double* G, const double* G_target, dou
     if (!(eps_h > 0.0)) return lsi::set_error(LS_ERR_INVALID, "eps_h must be > 0");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const AGeom g = make_ageom(p);
-    double* part = nullptr;
-    if (hipMalloc((void**)&part, (kLossBlocks + 1) * sizeof(double)) != hipSuccess)
-        return lsi::set_error(LS_ERR_NOMEM, "loss scratch allocation failed");
+    // the partials go to the solver's reduction scratch (allocated by ls_create)
+    size_t sbytes = 0;
+    double* part = static_cast<double*>(lsi::solver_scratch(s, &sbytes));
+    static_assert(kLossBlocks > 0, "loss blocks");
+    if (sbytes < (kLossBlocks + 1) * sizeof(double)) return lsi::set_error(LS_ERR_INVALID, "solver scratch too small");
     hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, st, g, G, G_target, eps_h, (double*)nullptr, part);
     hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, st, part, part + kLossBlocks);
     double h = 0.0;
     const bool ok = hipGetLastError() == hipSuccess &&
                     hipMemcpyAsync(&h, part + kLossBlocks, sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
                     hipStreamSynchronize(st) == hipSuccess;
-    (void)hipFree(part);
     if (!ok) return lsi::set_error(LS_ERR_DEVICE, "loss evaluation failed");
     *loss = h;
     return LS_OK;

@@ -152,11 +152,18 @@ struct ls_batch {
     int g_steps, g_shape;
     const void *g_G, *g_u, *g_v;
     hipStream_t cap;               // capture stream owned by the batch (the null stream cannot be captured)
+    hipEvent_t gdone;              // recorded after every launch of gexec
+    bool glaunched;                // gdone has been recorded since gexec was made
 };
 
 static int bfail(int code, const char* msg) { return lsi::set_error(code, msg); }
 
+// Retires the cached graph.  A launch of it may still be running on the
+// caller's stream (ls_batch_advance does not synchronise), so wait for the
+// event recorded after the last launch before destroying the executable.
 static void drop_graph(ls_batch* b) {
+    if (b->gexec && b->glaunched) (void)hipEventSynchronize(b->gdone);
+    b->glaunched = false;
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->gexec = nullptr;
@@ -260,6 +267,7 @@ int ls_batch_create(const ls_batch_member_t* members, int32_t count, int32_t mod
              hipMemcpy(b->dtiles[s], tl.data(), tl.size() * sizeof(BTile), hipMemcpyHostToDevice) == hipSuccess;
     }
     ok = ok && hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->gdone, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipMalloc((void**)&b->buf[0], b->total * sizeof(double)) == hipSuccess &&
          hipMalloc((void**)&b->buf[1], b->total * sizeof(double)) == hipSuccess &&
          hipMemset(b->buf[0], 0, b->total * sizeof(double)) == hipSuccess &&
@@ -280,6 +288,7 @@ int ls_batch_destroy(ls_batch_t* b) {
     (void)hipFree(b->buf[0]);
     (void)hipFree(b->buf[1]);
     if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->gdone) (void)hipEventDestroy(b->gdone);
     delete b;
     return LS_OK;
 }
@@ -343,6 +352,8 @@ int ls_batch_advance(ls_batch_t* b, double* G, const double* u, const double* v,
         b->g_v = v;
     }
     if (hipGraphLaunch(b->gexec, st) != hipSuccess) return bfail(LS_ERR_DEVICE, "hipGraphLaunch failed");
+    if (hipEventRecord(b->gdone, st) != hipSuccess) return bfail(LS_ERR_DEVICE, "hipEventRecord failed");
+    b->glaunched = true;
     return LS_OK;
 }
 

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "ls_api.h"
 #include "ls_internal.hpp"
@@ -379,6 +380,29 @@ __global__ void k_flame_project_final(int n, const double* __restrict__ part, do
     ga[q] = s;
 }
 
+// Per-block mode partials of ls_flame_project (kProjBlocks x kMaxModes
+// doubles, 64 KB): one buffer per device, allocated by the first call on that
+// device and kept until the process ends, so repeated projections (one per
+// gradient) neither allocate nor free.  Calls on one device share it; the
+// entry point synchronises its stream before returning, so calls from one
+// host thread never overlap.
+constexpr int kMaxProjDevices = 64;
+std::mutex g_proj_mu;
+double* g_proj_part[kMaxProjDevices] = {};
+
+int project_scratch(double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxProjDevices)
+        return lsi::set_error(LS_ERR_DEVICE, "no usable current device");
+    std::lock_guard<std::mutex> lock(g_proj_mu);
+    if (!g_proj_part[dev] && hipMalloc((void**)&g_proj_part[dev], kProjBlocks * kMaxModes * sizeof(double)) != hipSuccess) {
+        g_proj_part[dev] = nullptr;
+        return lsi::set_error(LS_ERR_NOMEM, "projection scratch allocation failed");
+    }
+    *out = g_proj_part[dev];
+    return LS_OK;
+}
+
 FlameModes draw_flame_modes(uint64_t seed, int n) {
     FlameModes m;
     std::memset(&m, 0, sizeof(m));
@@ -522,13 +546,12 @@ extern "C" int ls_flame_project(const ls_params_t* p, uint64_t mode_seed, int32_
     if (g_amps && nmodes > 0) {
         if (!gu || !gv || !gw) return lsi::set_error(LS_ERR_INVALID, "NULL velocity adjoint");
         double* part = nullptr;
-        if (hipMalloc((void**)&part, kProjBlocks * kMaxModes * sizeof(double)) != hipSuccess)
-            return lsi::set_error(LS_ERR_NOMEM, "projection scratch allocation failed");
+        rc = project_scratch(&part);
+        if (rc) return rc;
         hipLaunchKernelGGL(k_flame_project_modes, dim3(kProjBlocks), dim3(256), 0, st, c, p->nx, p->ny, p->nz,
                            draw_flame_modes(mode_seed, nmodes), gu, gv, gw, part);
         hipLaunchKernelGGL(k_flame_project_final, dim3(1), dim3(64), 0, st, (int)nmodes, part, g_amps);
         const bool ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
-        (void)hipFree(part);
         if (!ok) return lsi::set_error(LS_ERR_DEVICE, "projection launch failed");
     }
     return LS_OK;

@@ -49,6 +49,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define LSO_BC_PERIODIC 0
 #define LSO_BC_ZERO_GRAD 1
@@ -449,6 +452,47 @@ int64_t lso_advance(const lso_grid* g, double* G, double* u, double* v, double* 
     if (t_out) *t_out = t;
     if (dt_last) *dt_last = dt;
     return step;
+}
+
+double lso_l2(const lso_grid* g, const double* G, const double* G0);
+double lso_area(const lso_grid* g, const double* G);
+
+/* The reference's timed region (Local_code/level-set_MPI_2D/src/main.cpp:308-357,
+ * 3D :285-336) on `nthreads` OpenMP threads, for the CPU baseline: nsteps RK3
+ * steps of a fixed dt, HCR-2 after step > 0 with step % reinit_interval == 0,
+ * and every diag_interval steps the in-loop diagnostics the reference computes
+ * inside its timed region (L2 against G0 and the area / volume); no file
+ * output.  Returns the wall seconds of the loop (omp_get_wtime, the analogue
+ * of the reference's MPI_Wtime; with one process there is no max over ranks);
+ * diag[0], diag[1] receive the last L2 and area.  work: 6 padded doubles;
+ * flag: padded int32. */
+double lso_timed_loop(const lso_grid* g, double* G, const double* G0, double* u, double* v, double* w,
+                      double dt, int64_t nsteps, int32_t reinit_interval, int32_t diag_interval,
+                      int32_t nthreads, double* work, int32_t* flag, double* diag) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const double t0 = omp_get_wtime();
+#else
+    (void)nthreads;
+    const double t0 = 0.0;
+#endif
+    const int64_t n = lso_padded_size(g);
+    double* Gnew = work + 3 * n;
+    diag[0] = diag[1] = 0.0;
+    for (int64_t step = 0; step < nsteps; step++) {
+        lso_rk3_step(g, G, Gnew, u, v, w, NULL, dt, work);
+        memcpy(G, Gnew, (size_t)n * sizeof(double));
+        if (reinit_interval > 0 && step > 0 && step % reinit_interval == 0) lso_hcr2(g, G, work, flag);
+        if (diag_interval > 0 && (step + 1) % diag_interval == 0) {
+            diag[0] = lso_l2(g, G, G0);
+            diag[1] = lso_area(g, G);
+        }
+    }
+#ifdef _OPENMP
+    return omp_get_wtime() - t0;
+#else
+    return t0;
+#endif
 }
 
 /* ---- diagnostics (io.h:194-258) ------------------------------------------ */

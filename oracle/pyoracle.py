@@ -158,6 +158,22 @@ class Oracle(_Lib):
                                  C.byref(t), C.byref(dl))
         return G, n, t.value, dl.value
 
+    def timed_loop(self, g, G, G0, u, v, w=None, dt=1e-3, nsteps=10, reinit_interval=0, diag_interval=0,
+                   threads=1):
+        """lso_timed_loop: the reference's timed region on `threads` OpenMP
+        threads.  Returns (seconds, G_final, last_l2, last_area)."""
+        L = self.lib
+        L.lso_timed_loop.restype = C.c_double
+        L.lso_timed_loop.argtypes = [C.POINTER(Grid), _P, _P, _P, _P, _P, C.c_double, C.c_int64, C.c_int32,
+                                     C.c_int32, C.c_int32, _P, _I, _P]
+        G = G.copy()
+        work = np.zeros((6,) + G.shape)
+        fl = np.zeros(G.shape, np.int32)
+        d = np.zeros(2)
+        sec = L.lso_timed_loop(C.byref(g), _p(G), _p(G0), _p(u), _p(v), _p(w), dt, nsteps, reinit_interval,
+                               diag_interval, threads, _p(work), _pi(fl), _p(d))
+        return sec, G, float(d[0]), float(d[1])
+
     def deformation_velocity(self, g, t, T):
         u = np.zeros(g.shape)
         v = np.zeros(g.shape)

@@ -357,3 +357,23 @@ def test_live_reference_crossings(oracle, dim):
         a, b = oracle.crossings(g, p0), R.crossings(g, p0)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
         assert (a[1] != 0).sum() > 10
+
+
+@pytest.mark.parametrize("fma", [False, True])
+def test_timed_loop_is_the_advance_loop(fma):
+    """The CPU baseline's timed loop runs the same steps as the oracle's time
+    loop (HCR-2 after step > 0 with step % ri == 0), with the in-loop
+    diagnostics of the reference on top."""
+    o = Oracle(fma=fma)
+    g = make_grid(3, 12, 10, 9, bc=(0, 0, 0), s_l=0.1)
+    h = (g.dz, g.dy, g.dx)
+    G = o.fill_ghosts(g, bumpy_sdf(g.shape, h, 7))
+    vel = rand_vel(g.shape, 7, 0.5, n=3)
+    ref, n, _, _ = o.advance(g, G, *[a.copy() for a in vel], dt=2e-3, t_final=1e30, max_steps=11, reinit_interval=5)
+    for threads in (1, 3):
+        sec, got, l2, area = o.timed_loop(g, G, G, *[a.copy() for a in vel], dt=2e-3, nsteps=11, reinit_interval=5,
+                                          diag_interval=5, threads=threads)
+        assert sec > 0.0 and np.array_equal(got, ref)
+    # the last diagnostics were taken after step 10
+    mid, _, _, _ = o.advance(g, G, *[a.copy() for a in vel], dt=2e-3, t_final=1e30, max_steps=10, reinit_interval=5)
+    assert l2 == o.l2(g, mid, G) and area == o.area(g, mid)
