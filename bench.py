@@ -20,10 +20,12 @@ Prints ONE JSON line (rank 0).  Options:
                            1601^2 Zalesak and 64^3 deformation log workloads in
                            exact mode, --config zalesak) summaries of the
                            default line
-  --profile-pass           launch a 1 GiB copy, a ghost fill and stage kernels
-                           only (for rocprofv3 --pmc runs)
+  --profile-pass           launch the calibration triad (3 x 1 GiB) and the stage
+                           kernels only (for rocprofv3 --pmc runs)
   --summarize-pmc F W      turn two rocprofv3 --pmc CSV directories (FETCH_SIZE,
                            WRITE_SIZE) into profiles/stage_kernel_traffic_512_<mode>.json
+  --summarize-sq D...      SQ counters of the stage kernels from rocprofv3 --pmc
+                           CSV directories, per wave-plane, as a markdown table
   --config batch           the batched-ensemble workload instead: 96 2D
                            flames of seeded sizes in {128,192,256,384,512}^2,
                            S_L ~ U[0.05,0.2], 300 RK3 steps each, in one
@@ -479,20 +481,68 @@ def read_traffic(n, mode):
         return None
 
 
+def calib_lib():
+    """liblscalib.so: the triad / copy calibration kernels (built by build())."""
+    import ctypes as C
+    L = C.CDLL(os.path.join(ROOT, "level-set-method-code_amd", "liblscalib.so"))
+    L.lsc_triad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int64, C.c_void_p]
+    L.lsc_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    return L
+
+
+CALIB_N = 1 << 27  # doubles per array: 1 GiB each
+
+
+def triad_bandwidth(reps=10):
+    """Achievable HBM rate: the calibration triad c = a + s*b over three 1 GiB
+    arrays (2 GiB read, 1 GiB written per launch), best of `reps` launches timed
+    with HIP events on the launch stream.  Returns bytes/s."""
+    import ctypes as C
+    import torch
+    L = calib_lib()
+    a = torch.ones(CALIB_N, dtype=torch.float64, device="cuda")
+    b = torch.full_like(a, 0.5)
+    c = torch.empty_like(a)
+    st = torch.cuda.current_stream()
+    args = (C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.c_void_p(c.data_ptr()), 2.0, CALIB_N,
+            C.c_void_p(st.cuda_stream))
+    for _ in range(3):
+        assert L.lsc_triad(*args) == 0
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        assert L.lsc_triad(*args) == 0
+        e1.record(st)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    assert float(c[12345]) == 2.0
+    del a, b, c
+    torch.cuda.empty_cache()
+    return 24.0 * CALIB_N / best
+
+
 def profile_pass(ls, n, mode):
-    """Kernels for the --pmc runs: a 1 GiB device copy (the calibration kernel
-    of known traffic: 1 GiB read + 1 GiB write), one ghost fill and three
-    stage-2 launches."""
+    """Kernels for the --pmc runs: the calibration triad over 1 GiB arrays (the
+    kernel of known traffic: 2 GiB read + 1 GiB written), then one stage-1 and
+    three stage-2 launches."""
+    import ctypes as C
     import torch
     p, s, G, u, v, w, dt = setup(ls, n, mode)
-    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
+    L = calib_lib()
+    a = torch.ones(CALIB_N, dtype=torch.float64, device="cuda")
+    b = torch.ones_like(a)
+    c = torch.empty_like(a)
     torch.cuda.synchronize()
-    torch.mul(src, 2.0, out=dst)  # elementwise kernel: reads 1 GiB, writes 1 GiB
-    a, b = s.empty(), s.empty()
-    s.rk3_stage(1, G, G, a, u, v, w, dt=dt)
+    assert L.lsc_triad(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.c_void_p(c.data_ptr()), 2.0,
+                       CALIB_N, C.c_void_p(torch.cuda.current_stream().cuda_stream)) == 0
+    torch.cuda.synchronize()
+    del a, b, c
+    x, y = s.empty(), s.empty()
+    s.rk3_stage(1, G, G, x, u, v, w, dt=dt)
     for _ in range(3):
-        s.rk3_stage(2, a, G, b, u, v, w, dt=dt)
+        s.rk3_stage(2, x, G, y, u, v, w, dt=dt)
     torch.cuda.synchronize()
     print("profile pass done", flush=True)
 
@@ -507,7 +557,8 @@ def _pmc_rows(d):
 
 def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=None):
     """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) for the stage-2 kernel, each
-    scaled by what the same counter reads for the 1 GiB copy kernel."""
+    scaled by what the same counter reads for the calibration triad (2 GiB
+    read, 1 GiB written)."""
     def per_kernel(rows, counter):
         out = {}
         for r in rows:
@@ -517,21 +568,21 @@ def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=None):
         return out
     fr = per_kernel(_pmc_rows(fetch_dir), "FETCH_SIZE")
     wr = per_kernel(_pmc_rows(write_dir), "WRITE_SIZE")
-    copy_k = [k for k in fr if "MulFunctor" in k]
+    copy_k = [k for k in fr if "k_triad" in k]
     stage_k = [k for k in fr if ((mode == "fast" and FAST_STAGE_KERNEL in k) or
                                  (mode == "exact" and EXACT_STAGE_KERNEL in k))]
     if not copy_k or not stage_k:
         raise SystemExit("kernels not found in pmc output: %s" % list(fr))
     ck, sk = copy_k[0], stage_k[0]
     gib = float(1 << 30)
-    f_scale = gib / (fr[ck][0] * 1024.0)
+    f_scale = 2.0 * gib / (fr[ck][0] * 1024.0)
     w_scale = gib / (wr[ck][0] * 1024.0)
     fetch = sum(fr[sk]) / len(fr[sk]) * 1024.0 * f_scale
     write = sum(wr[sk]) / len(wr[sk]) * 1024.0 * w_scale
     cells = n ** 3
     d = {"kernel": sk, "mode": mode, "grid": [n, n, n],
          "fetch_kib_raw": sum(fr[sk]) / len(fr[sk]), "write_kib_raw": sum(wr[sk]) / len(wr[sk]),
-         "copy_fetch_kib_raw": fr[ck][0], "copy_write_kib_raw": wr[ck][0],
+         "calib_kernel": ck, "calib_fetch_kib_raw": fr[ck][0], "calib_write_kib_raw": wr[ck][0],
          "fetch_scale": f_scale, "write_scale": w_scale,
          "hbm_bytes_per_launch": fetch + write,
          "hbm_bytes_per_cell": (fetch + write) / cells,
@@ -541,6 +592,29 @@ def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=None):
     with open(out_path, "w") as fh:
         json.dump(d, fh, indent=1)
     print(json.dumps(d))
+
+
+def summarize_sq(dirs, n):
+    """SQ counters (rocprofv3 --pmc CSV directories) of the 3D stage kernels as
+    a markdown table, averaged over dispatches and divided by the wave-planes
+    of one launch (n^3 / 64 cells: one 64-wide wavefront row per plane)."""
+    vals = {}
+    for d in dirs:
+        for r in _pmc_rows(d):
+            k = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+            if "k_stage_zface" not in k:
+                continue
+            vals.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    wp = n ** 3 / 64.0
+    names = sorted({c for v in vals.values() for c in v})
+    print("| counter | " + " | ".join(sorted(vals)) + " |")
+    print("|---|" + "---|" * len(vals))
+    for c in names:
+        row = []
+        for k in sorted(vals):
+            x = vals[k].get(c)
+            row.append("%.2f" % (sum(x) / len(x) / wp) if x else "")
+        print("| %s per wave-plane | %s |" % (c, " | ".join(row)))
 
 
 def timed_steps(ls, n, mode, steps, warmup):
@@ -593,6 +667,7 @@ def main():
     ap.add_argument("--profile-pass", action="store_true")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--traffic-out", default=None)
+    ap.add_argument("--summarize-sq", nargs="+", metavar="PMC_DIR")
     ap.add_argument("--config", choices=["plane_modes", "batch", "adjoint", "sample", "zalesak"],
                     default="plane_modes")
     ap.add_argument("--sets", type=int, default=32)
@@ -603,6 +678,9 @@ def main():
 
     if a.summarize_pmc:
         summarize_pmc(a.summarize_pmc[0], a.summarize_pmc[1], a.n, a.mode, a.traffic_out)
+        return
+    if a.summarize_sq:
+        summarize_sq(a.summarize_sq, a.n)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus > 1 or world > 1:
@@ -630,6 +708,13 @@ def main():
         return
 
     value, elapsed, roof, dt, ok = timed_steps(ls, a.n, a.mode, a.steps, a.warmup)
+    # the achievable HBM rate on this part: the calibration triad (liblscalib.so)
+    bw = triad_bandwidth()
+    roof["achievable_peak"] = bw / 1e9
+    roof["achievable_kernel"] = "k_triad (c = a + s*b, 3 x 1 GiB FP64, 16-byte accesses)"
+    roof["frac_of_achievable"] = roof["achieved"] * 1e9 / bw
+    roof["step_achieved"] = value * STEP_BYTES / 1e9
+    roof["step_frac_of_achievable"] = value * STEP_BYTES / bw
     out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
            "steps": a.steps, "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
