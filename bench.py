@@ -13,7 +13,7 @@ Prints ONE JSON line (rank 0).  Options:
   --n N                    grid edge (default 512)
   --no-cpu-baseline        skip the CPU reference timing
   --no-adjoint             skip the reduced adjoint-vs-FD leg that the default
-                           line carries under "adjoint" (4 parameter sets)
+                           line carries under "adjoint" (16 parameter sets)
   --no-extra               skip the "other_mode" (the other arithmetic mode,
                            10 steps of the same workload), "batch" (the
                            batched ensemble below) and "reference_logs" (the
@@ -36,8 +36,10 @@ Prints ONE JSON line (rank 0).  Options:
                            reference's own sample code on 16 CPU threads
   --config adjoint         the adjoint workload instead: flame-inversion
                            gradients (128^3, 8^3 S_L nodes + 16 mode amplitudes,
-                           100 steps) for --sets seeded parameter sets vs
-                           central finite differences on the forward solver
+                           100 steps) for --sets seeded parameter sets (32),
+                           batched over --contexts concurrent adjoint contexts
+                           (8), vs central finite differences on the forward
+                           solver
   --config zalesak         the reference's own logged workloads instead: the 2D
                            Zalesak rotation at 201^2..1601^2 (S_L 0, no reinit),
                            end to end, against the ms/step of its RTX 4090 and
@@ -265,24 +267,37 @@ def bench_batch(ls, steps, mode, emit=True):
     return out
 
 
-def bench_adjoint(ls, mode, nsets, fd_sample=16, emit=True):
+def bench_adjoint(ls, mode, nsets, fd_sample=16, emit=True, nctx=8, serial_sets=4):
     """The adjoint workload: parameter gradients of the flame-inversion
-    loss for `nsets` seeded parameter sets vs central finite differences on
-    the plain forward solver (2 forward solves per parameter per gradient)."""
+    loss for `nsets` seeded parameter sets, batched over `nctx` concurrent
+    adjoint contexts (FlameBatch: ls_adjoint_run_sets), vs central finite
+    differences on the plain forward solver (2 forward solves per parameter
+    per gradient).  The first `serial_sets` sets also run one by one, for the
+    batching speedup and a bitwise check of the batch against them."""
     import numpy as np
     import torch
     m = ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT
     prob = ls.FlameInversion(n=128, nc=8, nmodes=16, nsteps=100, reinit_interval=5, ckpt=10, mode=m)
     sets = [prob.draw(100 + b) for b in range(nsets)]
+    nctx = max(1, min(nctx, nsets))
+    fb = ls.FlameBatch(prob, nctx)
+    fb.loss_and_grad(sets[:nctx])  # warm-up of every context
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grads = fb.loss_and_grad(sets)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / nsets
+    # the same sets one by one through the single context
+    ns = max(1, min(serial_sets, nsets))
     prob.loss_and_grad(*sets[0])  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    grads = []
-    for sl_c, amps in sets:
-        L, gc, ga = prob.loss_and_grad(sl_c, amps)
-        grads.append((L, gc, ga))
+    serial = [prob.loss_and_grad(*sets[q]) for q in range(ns)]
     torch.cuda.synchronize()
-    t_adj = (time.perf_counter() - t0) / nsets
+    t_serial = (time.perf_counter() - t0) / ns
+    same = all(serial[q][0] == grads[q][0] and torch.equal(serial[q][1], grads[q][1]) and
+               torch.equal(serial[q][2], grads[q][2]) for q in range(ns))
+    fb.close()
     # one plain forward solve + loss: the unit of finite differences
     reps = 8
     prob.loss_plain(*sets[0])
@@ -321,8 +336,11 @@ def bench_adjoint(ls, mode, nsets, fd_sample=16, emit=True):
            "config": {"workload": "3D 128^3 sphere flame r=0.2 in swirl + 16 seeded solenoidal modes, S_L on "
                                   "8^3 coarse nodes (trilinear), loss sum (H_e(G_T)-H_e(G_target))^2, "
                                   "e=1.5dx, HCR-2 every 5 steps, checkpoints every 10 steps",
-                      "parameter_sets": nsets, "parameters": npar, "mode": mode, "dt": prob.dt},
+                      "parameter_sets": nsets, "parameters": npar, "mode": mode, "dt": prob.dt,
+                      "concurrent_contexts": nctx},
            "adjoint_s_per_gradient": t_adj, "forward_solve_s": t_fwd,
+           "serial_s_per_gradient": t_serial, "batch_speedup_vs_serial": t_serial / t_adj,
+           "batch_bitwise_equal_to_serial": bool(same), "serial_sets_checked": ns,
            "fd_s_per_gradient": t_fd, "fd_gradients_per_s": 1.0 / t_fd, "speedup_vs_fd": t_fd / t_adj,
            "fd_sample": {"parameters": len(fd), "h": h, "max_rel_err": err},
            "adjoint_workspace_bytes": prob.adj.workspace_bytes(),
@@ -671,6 +689,7 @@ def main():
     ap.add_argument("--config", choices=["plane_modes", "batch", "adjoint", "sample", "zalesak"],
                     default="plane_modes")
     ap.add_argument("--sets", type=int, default=32)
+    ap.add_argument("--contexts", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other-mode and batched-ensemble summaries on the default line")
@@ -698,7 +717,7 @@ def main():
         bench_batch(ls, 300 if a.steps == 20 else a.steps, a.mode)
         return
     if a.config == "adjoint":
-        bench_adjoint(ls, a.mode, a.sets)
+        bench_adjoint(ls, a.mode, a.sets, nctx=a.contexts)
         return
     if a.config == "zalesak":
         bench_zalesak(ls, a.mode)
@@ -728,11 +747,14 @@ def main():
         out["cpu_baseline"] = cpu_baseline(ls, a.n)
     if not a.no_adjoint:
         # second half of the metric: adjoint vs finite-difference gradients (reduced batch)
-        adj = bench_adjoint(ls, a.mode, nsets=4, fd_sample=8, emit=False)
+        adj = bench_adjoint(ls, a.mode, nsets=16, fd_sample=8, emit=False, nctx=8, serial_sets=2)
         out["adjoint"] = {k: adj[k] for k in ("value", "unit", "adjoint_s_per_gradient", "forward_solve_s",
-                                               "fd_s_per_gradient", "speedup_vs_fd", "fd_sample")}
+                                               "fd_s_per_gradient", "speedup_vs_fd", "fd_sample",
+                                               "serial_s_per_gradient", "batch_speedup_vs_serial",
+                                               "batch_bitwise_equal_to_serial")}
         out["adjoint"]["workload"] = adj["config"]["workload"]
-        out["adjoint"]["parameter_sets"] = 4
+        out["adjoint"]["parameter_sets"] = 16
+        out["adjoint"]["concurrent_contexts"] = adj["config"]["concurrent_contexts"]
     if not a.no_extra:
         # the other arithmetic mode on the same workload (short run), and the batched 2D ensemble
         other = "exact" if a.mode == "fast" else "fast"

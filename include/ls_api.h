@@ -321,6 +321,29 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
                    double eps_h, double dt, double* loss, double* gG0, double* gu, double* gv,
                    double* gw, double* gsl, void* stream);
 
+/* Many parameter sets at once (no counterpart in the reference): set q runs
+ * ls_adjoint_run on context q % nctx, with that context's stream
+ * (streams[c], NULL = all on the null stream).  Each context gets a host
+ * thread of its own and takes its sets in order, so up to nctx sets run
+ * concurrently on the GPU; results are those of ls_adjoint_run called set by
+ * set.  The contexts must be distinct and bound to distinct solvers; sets
+ * sharing a context must not share output buffers with sets of another
+ * context that run at the same time.  Returns the error of the first failing
+ * set (its index in the message).  Synchronises every stream used. */
+typedef struct ls_adjoint_set {
+    const double* G0;
+    const double* u;
+    const double* v;
+    const double* w;
+    const double* sl_field;     /* NULL: the solver's scalar s_l */
+    const double* G_target;
+    double eps_h, dt;
+    double* loss;               /* host */
+    double *gG0, *gu, *gv, *gw, *gsl;   /* device, or all NULL (loss only) */
+} ls_adjoint_set_t;
+int ls_adjoint_run_sets(ls_adjoint_t* const* ctx, void* const* streams, int32_t nctx,
+                        const ls_adjoint_set_t* sets, int32_t nsets);
+
 /* Parameter maps of the flame-inversion problem (no counterpart in the
  * reference).  S_L(x) is the trilinear interpolant of nc^3 coarse node values
  * (device, x fastest) over the interior box; the velocity is the 3D swirl

@@ -67,6 +67,14 @@ class BatchMember(C.Structure):
                 ("dx", C.c_double), ("dy", C.c_double), ("s_l", C.c_double), ("dt", C.c_double)]
 
 
+class AdjointSet(C.Structure):
+    """Mirror of ``ls_adjoint_set_t``."""
+    _fields_ = [("G0", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+                ("sl_field", C.c_void_p), ("G_target", C.c_void_p), ("eps_h", C.c_double), ("dt", C.c_double),
+                ("loss", C.POINTER(C.c_double)), ("gG0", C.c_void_p), ("gu", C.c_void_p), ("gv", C.c_void_p),
+                ("gw", C.c_void_p), ("gsl", C.c_void_p)]
+
+
 class LSError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("ls error %d: %s" % (code, msg))
@@ -128,6 +136,7 @@ def lib():
     L.ls_adjoint_loss.argtypes = [vp, vp, vp, C.c_double, C.POINTER(C.c_double), vp]
     L.ls_adjoint_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double,
                                  C.POINTER(C.c_double), vp, vp, vp, vp, vp, vp]
+    L.ls_adjoint_run_sets.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int32, C.POINTER(AdjointSet), C.c_int32]
     L.ls_flame_fields.argtypes = [P, C.c_uint64, C.c_int32, vp, C.c_int32, vp, vp, vp, vp, vp, vp]
     L.ls_flame_project.argtypes = [P, C.c_uint64, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
     L.ls_sample_field_size.argtypes = [C.c_int32, C.c_int32]
@@ -508,6 +517,7 @@ class FlameInversion:
         self.mode_seed = mode_seed
         self.eps_h = eps_factor * self.p.dx
         self.solver = Solver(self.p)
+        self.ckpt = ckpt
         self.adj = Adjoint(self.solver, nsteps, ckpt)
         G0, _, _, _ = init_case(self.p, CASE_SPHERE_SWIRL3D)
         self.solver.fill_ghosts(G0)
@@ -568,6 +578,67 @@ class FlameInversion:
     def close(self):
         self.adj.close()
         self.solver.close()
+
+
+class FlameBatch:
+    """Gradients of a :class:`FlameInversion` for many parameter sets at once.
+    ``nctx`` workers, each with its own solver, adjoint context, parameter
+    fields, gradient fields and stream, take the sets in waves of ``nctx``:
+    the maps to fields on each worker's stream, then one ls_adjoint_run_sets
+    call (one host thread per context inside the library, so the sets of a
+    wave run concurrently on the GPU), then the transposed maps.  Results are
+    those of ``problem.loss_and_grad`` set by set."""
+
+    def __init__(self, problem, nctx=4):
+        import torch
+        self.prob = problem
+        self.workers = []
+        for _ in range(nctx):
+            s = Solver(problem.p)
+            w = dict(solver=s, adj=Adjoint(s, problem.nsteps, problem.ckpt), stream=torch.cuda.Stream(),
+                     fields=[s.empty() for _ in range(4)],
+                     grads={k: s.empty() for k in ("u", "v", "w", "sl")}, loss=C.c_double())
+            self.workers.append(w)
+        self.ctx = (C.c_void_p * nctx)(*[w["adj"].h.value for w in self.workers])
+        self.streams = (C.c_void_p * nctx)(*[w["stream"].cuda_stream for w in self.workers])
+
+    def loss_and_grad(self, sets):
+        """sets: list of (sl_coarse, amps).  Returns a list of (loss, dL/d sl_coarse, dL/d amps)."""
+        import torch
+        P = self.prob
+        nctx = len(self.workers)
+        out = []
+        for w0 in range(0, len(sets), nctx):
+            wave = sets[w0:w0 + nctx]
+            arr = (AdjointSet * len(wave))()
+            main = torch.cuda.current_stream()
+            for c, (sl_c, amps) in enumerate(wave):
+                wk = self.workers[c]
+                wk["stream"].wait_stream(main)
+                sl, u, v, w = wk["fields"]
+                with torch.cuda.stream(wk["stream"]):
+                    flame_fields(P.p, P.mode_seed, sl_c.contiguous(), amps.contiguous(), sl, u, v, w,
+                                 stream=wk["stream"].cuda_stream)
+                g = wk["grads"]
+                arr[c] = AdjointSet(P.G0.data_ptr(), u.data_ptr(), v.data_ptr(), w.data_ptr(), sl.data_ptr(),
+                                    P.G_target.data_ptr(), P.eps_h, P.dt, C.pointer(wk["loss"]), None,
+                                    g["u"].data_ptr(), g["v"].data_ptr(), g["w"].data_ptr(), g["sl"].data_ptr())
+            check(lib().ls_adjoint_run_sets(self.ctx, self.streams, nctx, arr, len(wave)))
+            for c in range(len(wave)):
+                wk = self.workers[c]
+                g = wk["grads"]
+                with torch.cuda.stream(wk["stream"]):
+                    gc, ga = flame_project(P.p, P.mode_seed, P.nc, P.nmodes, g["sl"], g["u"], g["v"], g["w"],
+                                           stream=wk["stream"].cuda_stream)
+                main.wait_stream(wk["stream"])
+                out.append((wk["loss"].value, gc, ga))
+        return out
+
+    def close(self):
+        for w in self.workers:
+            w["adj"].close()
+            w["solver"].close()
+        self.workers = []
 
 
 def _flame_loss_function():

@@ -33,6 +33,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "ls_api.h"
@@ -848,6 +851,57 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
     return LS_OK;
 #undef ADJ_HIP
 #undef ADJ_RC
+}
+
+int ls_adjoint_run_sets(ls_adjoint_t* const* ctx, void* const* streams, int32_t nctx, const ls_adjoint_set_t* sets,
+                        int32_t nsets) {
+    if (!ctx || !sets || nctx < 1 || nsets < 0) return lsi::set_error(LS_ERR_INVALID, "bad context or set list");
+    for (int c = 0; c < nctx; c++) {
+        if (!ctx[c]) return lsi::set_error(LS_ERR_INVALID, "NULL adjoint context");
+        for (int d = 0; d < c; d++)
+            if (ctx[d] == ctx[c] || ctx[d]->s == ctx[c]->s)
+                return lsi::set_error(LS_ERR_INVALID, "contexts of one call must not share a context or solver");
+    }
+    auto run_one = [&](int c, int q) {
+        const ls_adjoint_set_t& e = sets[q];
+        return ls_adjoint_run(ctx[c], e.G0, e.u, e.v, e.w, e.sl_field, e.G_target, e.eps_h, e.dt, e.loss, e.gG0,
+                              e.gu, e.gv, e.gw, e.gsl, streams ? streams[c] : nullptr);
+    };
+    const int nthr = std::min<int>(nctx, nsets);
+    if (nthr <= 1) {
+        for (int q = 0; q < nsets; q++) {
+            const int rc = run_one(0, q);
+            if (rc) return rc;
+        }
+        return LS_OK;
+    }
+    // One host thread per context; context c takes sets c, c + nctx, ... in
+    // order on its own stream.  The first failure (lowest set index) is
+    // reported with its message.
+    std::mutex mu;
+    int err_q = nsets, err_rc = LS_OK;
+    std::string err_msg;
+    std::vector<std::thread> pool;
+    for (int c = 0; c < nthr; c++) {
+        pool.emplace_back([&, c]() {
+            if (hipSetDevice(lsi::solver_device(ctx[c]->s)) != hipSuccess) {
+                std::lock_guard<std::mutex> lock(mu);
+                if (c < err_q) { err_q = c; err_rc = LS_ERR_DEVICE; err_msg = "hipSetDevice failed in a set worker"; }
+                return;
+            }
+            for (int q = c; q < nsets; q += nctx) {
+                const int rc = run_one(c, q);
+                if (rc) {
+                    std::lock_guard<std::mutex> lock(mu);
+                    if (q < err_q) { err_q = q; err_rc = rc; err_msg = ls_last_error(); }
+                    return;
+                }
+            }
+        });
+    }
+    for (auto& t : pool) t.join();
+    if (err_rc) return lsi::set_error(err_rc, ("parameter set " + std::to_string(err_q) + ": " + err_msg).c_str());
+    return LS_OK;
 }
 
 }  // extern "C"

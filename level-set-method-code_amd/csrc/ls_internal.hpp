@@ -16,6 +16,8 @@ const ls_params_t& solver_params(const ls_solver_t* s);
 // in bytes.  Entry points that take the handle may use it between their own
 // launches; the one-handle-per-thread rule keeps uses apart.
 void* solver_scratch(ls_solver_t* s, size_t* bytes);
+// The HIP device a solver was created on.
+int solver_device(const ls_solver_t* s);
 // Deformation velocity of the 3D sphere test at time t (ls_cases.hip).
 int launch_deformation_velocity(const ls_params_t& p, double t, double period, double* u,
                                 double* v, double* w, hipStream_t st);

@@ -761,6 +761,7 @@ const char* ls_last_error(void) { return g_err.c_str(); }
 namespace lsi {
 int set_error(int code, const char* msg) { return fail(code, msg); }
 const ls_params_t& solver_params(const ls_solver_t* s) { return s->p; }
+int solver_device(const ls_solver_t* s) { return s->device; }
 void* solver_scratch(ls_solver_t* s, size_t* bytes) {
     *bytes = (kRedBlocks + 1) * sizeof(DiagPart);
     return s->diag;

@@ -224,6 +224,38 @@ def test_flame_autograd_function(ls, cuda):
     prob.close()
 
 
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_flame_batch_bitwise_per_set(ls, cuda, mode):
+    """ls_adjoint_run_sets through FlameBatch: 7 parameter sets over 3
+    concurrent contexts (three waves, the last one partial) give for every set
+    the loss and gradients of loss_and_grad run set by set, bit for bit."""
+    torch = cuda
+    m = ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT
+    prob = ls.FlameInversion(n=32, nc=4, nmodes=6, nsteps=12, reinit_interval=5, ckpt=5, mode=m)
+    sets = [prob.draw(200 + q) for q in range(7)]
+    fb = ls.FlameBatch(prob, 3)
+    got = fb.loss_and_grad(sets)
+    for q, (sl_c, amps) in enumerate(sets):
+        L, gc, ga = prob.loss_and_grad(sl_c, amps)
+        assert got[q][0] == L, "set %d loss" % q
+        assert torch.equal(got[q][1], gc) and torch.equal(got[q][2], ga), "set %d gradient" % q
+    assert len({g[0] for g in got}) == len(sets)  # the sets differ
+    # a second call on the same contexts reuses their workspace
+    again = fb.loss_and_grad(sets[:2])
+    assert again[0][0] == got[0][0] and torch.equal(again[1][1], got[1][1])
+    fb.close()
+    prob.close()
+
+
+def test_adjoint_run_sets_rejects_shared_context(ls, cuda):
+    import ctypes as C
+    prob = ls.FlameInversion(n=16, nc=2, nmodes=2, nsteps=3, reinit_interval=5, ckpt=3)
+    ctx = (C.c_void_p * 2)(prob.adj.h.value, prob.adj.h.value)
+    sets = (ls.AdjointSet * 1)()
+    assert ls.lib().ls_adjoint_run_sets(ctx, None, 2, sets, 1) == ls.LS_ERR_INVALID
+    prob.close()
+
+
 @pytest.mark.slow
 def test_flame_full_size_gradient_vs_central_differences(ls, cuda):
     """The adjoint workload at full size: 128^3, 8^3 S_L nodes + 16 modes, 100
