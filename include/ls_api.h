@@ -276,11 +276,12 @@ int ls_batch_fill_ghosts(ls_batch_t* b, double* G, void* stream);
  * captures and replays a hipGraph.  Does not synchronise. */
 int ls_batch_advance(ls_batch_t* b, double* G, const double* u, const double* v, int32_t nsteps,
                      int32_t use_graph, void* stream);
-/* Schedule shapes 0..3: the y-march, one wavefront per 64-column strip and
- * chunk of 32, 16, 4 or 8 rows (both modes).  ls_batch_autotune times
- * probe_steps steps of each (two passes) on a scratch copy and keeps the
- * fastest (*best_ms = its time per step).  The result does not depend on the
- * shape.  Synchronises the stream. */
+/* Schedule shapes 0..7: the y-march, one wavefront per 64-column strip and
+ * chunk of 32, 16, 4 or 8 rows (shape % 4, both modes), with the kernel's own
+ * register budget (shapes 0-3) or the budget of one more wavefront per SIMD
+ * (shapes 4-7).  ls_batch_autotune times probe_steps steps of each (two
+ * passes) on a scratch copy and keeps the fastest (*best_ms = its time per
+ * step).  The result does not depend on the shape.  Synchronises the stream. */
 int ls_batch_set_tile(ls_batch_t* b, int32_t shape);
 int32_t ls_batch_get_tile(const ls_batch_t* b);
 int ls_batch_autotune(ls_batch_t* b, const double* G, const double* u, const double* v,

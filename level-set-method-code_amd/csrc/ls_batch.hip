@@ -46,21 +46,28 @@ struct BTile {
     int32_t member, tx, ty, pad;
 };
 
-// Schedule shapes: y-march chunks of kChunks[shape] rows (ls_ymarch.hpp), one
-// wavefront per 64-column strip and chunk.
+// Schedule shapes: y-march chunks of kChunks[shape % kNumChunks] rows
+// (ls_ymarch.hpp), one wavefront per 64-column strip and chunk, compiled with
+// the register budget the compiler picks (shape < kNumChunks) or with the
+// budget of one more wavefront per SIMD (MoreWaves: the fast kernel needs 86
+// VGPRs, 5 waves, and gets 6 at 80; the exact one needs 147, 3 waves, and
+// gets 4 at 128), at the price of recomputed or spilled values.  The result
+// does not depend on the shape.
 constexpr int kChunks[] = {32, 16, 4, 8};
-constexpr int kNumShapes = 4;
+constexpr int kNumChunks = 4;
+constexpr int kNumShapes = 2 * kNumChunks;
+template <bool EXACT>
+struct MoreWaves {
+    static constexpr int value = EXACT ? 4 : 6;
+};
 
 // Units (member, strip, chunk of ch rows), one wavefront each; fast or exact
-// arithmetic.
-template <int STAGE, bool EXACT>
-__global__ __launch_bounds__(lsf::kYW) void k_batch_ymarch(const BMember* __restrict__ mem,
-                                                           const BTile* __restrict__ tiles, int ntiles, int ch,
-                                                           const double* __restrict__ Gs,
-                                                           const double* __restrict__ Gn,
-                                                           double* __restrict__ out,
-                                                           const double* __restrict__ u,
-                                                           const double* __restrict__ v) {
+// arithmetic; MW = 0: the compiler's register budget, 1: one more wave.
+template <int STAGE, bool EXACT, int MW>
+__global__ __launch_bounds__(lsf::kYW) __attribute__((amdgpu_waves_per_eu(MW ? MoreWaves<EXACT>::value : 1)))
+void k_batch_ymarch(const BMember* __restrict__ mem, const BTile* __restrict__ tiles, int ntiles, int ch,
+                    const double* __restrict__ Gs, const double* __restrict__ Gn, double* __restrict__ out,
+                    const double* __restrict__ u, const double* __restrict__ v) {
     __shared__ lsf::YLds L;
     // one unit per block, dealt XCD-contiguously (block b runs on XCD b % 8):
     // XCD e takes units [e*per, (e+1)*per), so the consecutive chunks of a
@@ -121,15 +128,21 @@ __global__ __launch_bounds__(256) void k_copy(const double* __restrict__ a, doub
 typedef void (*StageFn)(const BMember*, const BTile*, int, int, const double*, const double*, double*,
                         const double*, const double*);
 
-void stage_fns(int mode, StageFn* f) {
+template <bool EXACT, int MW>
+void stage_fns_t(StageFn* f) {
+    f[0] = k_batch_ymarch<1, EXACT, MW>;
+    f[1] = k_batch_ymarch<2, EXACT, MW>;
+    f[2] = k_batch_ymarch<3, EXACT, MW>;
+}
+
+void stage_fns(int mode, int shape, StageFn* f) {
+    const bool more = shape >= kNumChunks;
     if (mode == LS_MODE_FAST) {
-        f[0] = k_batch_ymarch<1, false>;
-        f[1] = k_batch_ymarch<2, false>;
-        f[2] = k_batch_ymarch<3, false>;
+        if (more) stage_fns_t<false, 1>(f);
+        else stage_fns_t<false, 0>(f);
     } else {
-        f[0] = k_batch_ymarch<1, true>;
-        f[1] = k_batch_ymarch<2, true>;
-        f[2] = k_batch_ymarch<3, true>;
+        if (more) stage_fns_t<true, 1>(f);
+        else stage_fns_t<true, 0>(f);
     }
 }
 
@@ -143,8 +156,8 @@ struct ls_batch {
     std::vector<int64_t> offs;
     std::vector<BMember> hmem;
     BMember* dmem;
-    BTile* dtiles[kNumShapes];
-    int ntiles[kNumShapes];
+    BTile* dtiles[kNumChunks];     // unit list per chunk length
+    int ntiles[kNumChunks];
     double* buf[2];
     // cached graph of the last captured sequence
     hipGraphExec_t gexec;
@@ -173,15 +186,16 @@ static void drop_graph(ls_batch* b) {
 // Enqueue nsteps RK3 steps on `st` (no synchronisation).
 static int enqueue_steps(ls_batch* b, double* G, const double* u, const double* v, int nsteps, hipStream_t st) {
     StageFn f[3];
-    stage_fns(b->mode, f);
-    const int nt = b->ntiles[b->shape];
-    const int ch = kChunks[b->shape];
+    stage_fns(b->mode, b->shape, f);
+    const int ci = b->shape % kNumChunks;
+    const int nt = b->ntiles[ci];
+    const int ch = kChunks[ci];
     const unsigned grid = (unsigned)((nt + 7) / 8 * 8);
     const dim3 block(lsf::kYW, 1, 1);
     double* cur = G;
     double* a = b->buf[0];
     double* c = b->buf[1];
-    const BTile* tl = b->dtiles[b->shape];
+    const BTile* tl = b->dtiles[ci];
     for (int s = 0; s < nsteps; s++) {
         hipLaunchKernelGGL(f[0], dim3(grid), block, 0, st, b->dmem, tl, nt, ch, cur, cur, a, u, v);
         hipLaunchKernelGGL(f[1], dim3(grid), block, 0, st, b->dmem, tl, nt, ch, a, cur, c, u, v);
@@ -246,7 +260,7 @@ int ls_batch_create(const ls_batch_member_t* members, int32_t count, int32_t mod
     b->total = off;
     bool ok = hipMalloc((void**)&b->dmem, count * sizeof(BMember)) == hipSuccess &&
               hipMemcpy(b->dmem, b->hmem.data(), count * sizeof(BMember), hipMemcpyHostToDevice) == hipSuccess;
-    for (int s = 0; ok && s < kNumShapes; s++) {
+    for (int s = 0; ok && s < kNumChunks; s++) {
         std::vector<BTile> tl;
         // largest members first: they carry the most tiles and start earliest
         std::vector<int> order(count);
@@ -284,7 +298,7 @@ int ls_batch_destroy(ls_batch_t* b) {
     if (!b) return LS_OK;
     drop_graph(b);
     (void)hipFree(b->dmem);
-    for (int s = 0; s < kNumShapes; s++) (void)hipFree(b->dtiles[s]);
+    for (int s = 0; s < kNumChunks; s++) (void)hipFree(b->dtiles[s]);
     (void)hipFree(b->buf[0]);
     (void)hipFree(b->buf[1]);
     if (b->cap) (void)hipStreamDestroy(b->cap);

@@ -81,12 +81,13 @@ def test_batch_graph_replays(ls, cuda):
 
 
 def test_batch_exact_all_shapes_bitwise(ls, oracle, cuda):
-    """Exact mode on every chunk length: bitwise the oracle each time."""
+    """Exact mode on every schedule shape (chunk length x register budget):
+    bitwise the oracle each time."""
     b, G, U, V, inputs = _build(ls, cuda, ls.MODE_EXACT)
     steps = 5
     refs = [oracle.advance(g, Gh, uh, vh, dt=dt, t_final=1e30, max_steps=steps)[0]
             for (g, Gh, uh, vh, dt) in inputs]
-    for shape in range(4):
+    for shape in range(8):
         Gs = G.clone()
         b.set_tile(shape)
         b.advance(Gs, U, V, steps, graph=False)
@@ -127,14 +128,14 @@ def test_batch_autotune_keeps_results(ls, cuda):
     b.set_tile(0)
     b.advance(G, U, V, 5, graph=False)
     shape, ms = b.autotune(G2, U, V, probe_steps=2)
-    assert 0 <= shape < 4 and ms > 0
+    assert 0 <= shape < 8 and ms > 0
     b.advance(G2, U, V, 5, graph=True)
     cuda.cuda.synchronize()
     assert cuda.equal(G, G2)  # the arithmetic does not depend on the chunk length
 
 
 def test_batch_fast_shapes_match_serial(ls, cuda):
-    """Fast mode: every schedule shape (y-march chunk length) gives the same
+    """Fast mode: every schedule shape (chunk length, register budget) gives the same
     bits, and so does the single-grid solver run member by member: one thread
     per cell for the small members, its own y-march for the tall one."""
     spec = MEMBERS + [(130, 5500, (0, 1), 0.1)]
@@ -155,13 +156,13 @@ def test_batch_fast_shapes_match_serial(ls, cuda):
     b.fill_ghosts(G)
     steps = 6
     results = []
-    for shape in range(4):
+    for shape in range(8):
         Gs = G.clone()
         b.set_tile(shape)
         b.advance(Gs, U, V, steps, graph=False)
         results.append(Gs)
     cuda.cuda.synchronize()
-    for shape in range(1, 4):
+    for shape in range(1, 8):
         assert cuda.equal(results[0], results[shape]), "shape %d" % shape
     for q, p in enumerate(params):
         s = ls.Solver(p)
