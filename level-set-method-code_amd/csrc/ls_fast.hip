@@ -562,6 +562,10 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
             }
         }
         __syncthreads();  // faces
+        // Plane k+1 goes into sG (free after the face barrier) before this
+        // plane's stores are issued: vmcnt counts stores too, so the wait for
+        // the prefetched halo would otherwise also wait for them.
+        if (k + 1 < ke) put_plane(zG[4], nx);
         // --- derivatives, RHS, stage combine
         if (active && EXACT) {
             // dweno_*_3d: (face(c+1/2) - face(c-1/2)) / h; rhs and stage in the
@@ -607,7 +611,6 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         }
         zlo = zhi;
         zlo_L = (int)Lz;
-        if (k + 1 < ke) put_plane(zG[4], nx);  // after the face barrier: sG is free
 #pragma unroll
         for (int q = 0; q < 6; q++) zG[q] = zG[q + 1];
 #pragma unroll
@@ -842,11 +845,10 @@ __global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __re
     };
 
     double val[kQPer];
-    for (int kk = kb - 2; kk <= kb + 1; kk++) {
+    for (int kk = kb - 2; kk <= kb + 2; kk++) {
         load_q(kk, val);
         put_q(kk, val);
     }
-    load_q(kb + 2, val);
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < kP1Per; a++)
@@ -854,13 +856,16 @@ __global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __re
     double fm = phi1(zev(kb - 1), lx + 2, ly + 2);  // phi_1 of this column one plane below
 
     for (int k = kb; k < ke; k++) {
-        put_q(k + 2, val);  // slot of plane k-3, last read two planes ago
         if (k + 1 < ke) load_q(k + 3, val);
         __syncthreads();  // phi_0 plane k+2 in place; phi_1 plane k-1 no longer read
 #pragma unroll
         for (int a = 0; a < kP1Per; a++)
             if (tid + a * HX * HY < kP1Items) sP1[(k + 1) & 1][py[a]][px[a]] = phi1(zev(k + 1), ex[a], ey[a]);
         __syncthreads();  // phi_1 plane k+1 in place
+        // phi_0 plane k+3 into the slot of plane k-2, last read in the previous
+        // plane, before this plane's stores: vmcnt counts stores too, so a wait
+        // for the loads placed after them would also wait for the stores.
+        if (k + 1 < ke) put_q(k + 3, val);
         if (active) {
             const int s0 = k & 1, X = lx + 1, Y = ly + 1;
             const double fc = sP1[s0][Y][X];
