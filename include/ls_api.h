@@ -206,6 +206,20 @@ int ls_check_finite(ls_solver_t* s, const double* G, void* stream);
  * two host threads) race on it.  Rejects the parameters ls_create rejects. */
 int ls_diag_noalloc(const ls_params_t* p, const double* G, const double* G0, double* out, void* stream);
 
+/* Device check of the exact mode's division sequences (no counterpart in the
+ * reference; the tests call it): out[i] = a[i] / b[i] as the exact-mode
+ * kernels compute it on the GPU, for device arrays of n doubles.
+ *   kind 0: rdiv_u, the sequence used for quotients with a divisor of their
+ *           own (b in [2^-1000, 2^1000], a = 0 or |a| in [2^-960, 2^960],
+ *           |a|/b in [2^-1000, 2^700]);
+ *   kind 1: qdiv_u with y = rdiv_u(1, b), the sequence of quotients sharing a
+ *           divisor (b in [2^-300, 2^300], a = 0 or |a| in [2^-960, 2^960],
+ *           |a|/b in [2^-1000, 2^1000]);
+ *   kind 2: div6_u, a / 6 (b unused; a = 0 or |a| in [2^-960, 2^1000]).
+ * Inside those ranges each must equal the IEEE quotient bit for bit.
+ * Stream-ordered, no synchronisation. */
+int ls_div_check(const double* a, const double* b, double* out, int64_t n, int32_t kind, void* stream);
+
 /* Write a HOST copy of a padded field in the reference's binary G format:
  * int32 {nx, ny, nghost} (2D) or {nx, ny, nz, nghost} (3D), then the padded
  * doubles with the y-ghost rows replaced by the adjacent interior row. */

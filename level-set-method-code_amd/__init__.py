@@ -111,6 +111,7 @@ def lib():
                              C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double), vp]
     L.ls_diag.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]
     L.ls_check_finite.argtypes = [vp, vp, vp]
+    L.ls_div_check.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, vp]
     L.ls_diag_noalloc.argtypes = [P, vp, vp, vp, vp]
     L.ls_write_G.argtypes = [P, vp, C.c_char_p]
     L.ls_write_G_vtk.argtypes = [P, vp, C.c_char_p]
@@ -307,6 +308,22 @@ def diag_noalloc(p, G, G0=None, out=None, stream=None):
     if not out.is_cuda or out.dtype != torch.float64 or out.numel() < 3 or not out.is_contiguous():
         raise ValueError("out must be a contiguous float64 device tensor of 3 elements")
     check(lib().ls_diag_noalloc(C.byref(p), _ptr(G), _ptr(G0), _ptr(out), _stream(stream)))
+    return out
+
+
+def div_check(a, b, kind, stream=None):
+    """ls_div_check: the exact mode's division sequence ``kind`` (0 rdiv_u,
+    1 qdiv_u on the reciprocal rdiv_u(1, b), 2 div6_u, which ignores b) on
+    contiguous float64 device tensors; returns the quotients as a new device
+    tensor."""
+    import torch
+    for t in (a, b):
+        if t is not None and (not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous()):
+            raise ValueError("operands must be contiguous float64 device tensors")
+    if b is not None and b.numel() != a.numel():
+        raise ValueError("a and b differ in length")
+    out = torch.empty_like(a)
+    check(lib().ls_div_check(_ptr(a), _ptr(b), _ptr(out), a.numel(), kind, _stream(stream)))
     return out
 
 

@@ -115,4 +115,41 @@ LS_CRDIV_FN double dq(double a, double b, double y) {
     else return qdiv_u(a, b, y);
 }
 
+#if defined(__HIPCC__)
+// RN(a/b) for a quotient with a divisor of its own (the WENO weights'
+// numerators d_k / (eps + beta_k)^2, and the reciprocals 1/s and 1/|grad G|
+// that qdiv_u takes as y).  It is the sequence the compiler emits for a / b
+// on gfx950 -- v_rcp_f64, two Newton steps, the quotient and one residual
+// correction -- without the range scaling (two v_div_scale) and the
+// special-case fix-up (v_div_fixup) around it.  v_div_scale returns its
+// operand unchanged, and leaves VCC clear so that v_div_fmas is a plain FMA,
+// unless b or 1/b is subnormal, a/b is subnormal, a is zero or below
+// 2^-969, or the exponent of a exceeds that of b by 768 or more;
+// v_div_fixup returns the quotient unchanged for finite nonzero operands
+// whose quotient is normal.  Inside those ranges both sequences compute the
+// same values, so this one returns the IEEE quotient too.  A zero numerator
+// (which v_div_scale hands to v_div_fixup) keeps its sign through the
+// correction's form (b q - a)(-y), as in qdiv_u.  Range: b in
+// [2^-1000, 2^1000], a = 0 or |a| in [2^-960, 2^960], |a|/b in
+// [2^-1000, 2^700]; the exact-mode uses stay far inside it (divisors
+// (eps + beta)^2 in [2^-40, 2^214], weight sums in [2^-218, 2^42], |grad G|
+// in [2^-100, 2^117]; numerators 0.1, 0.3, 0.6 and 1).  Device only.
+__device__ __forceinline__ double rdiv_u(double a, double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    const double q = a * y;
+    return fma(fma(b, q, -a), -y, q);
+}
+
+// a / b in the flavour D: the IEEE division, or rdiv_u.
+template <int D>
+__device__ __forceinline__ double dr(double a, double b) {
+    if constexpr (D == kDivIEEE) return a / b;
+    else return rdiv_u(a, b);
+}
+#endif
+
 }  // namespace lsx

@@ -51,11 +51,11 @@ __device__ __forceinline__ double weno_l(double v0, double v1, double v2, double
     double b0 = beta3(v0, v1, v2);
     double b1 = beta3(v1, v2, v3);
     double b2 = beta3(v2, v3, v4);
-    double a0 = 0.1 / ((kWenoEps + b0) * (kWenoEps + b0));
-    double a1 = 0.6 / ((kWenoEps + b1) * (kWenoEps + b1));
-    double a2 = 0.3 / ((kWenoEps + b2) * (kWenoEps + b2));
+    double a0 = dr<D>(0.1, (kWenoEps + b0) * (kWenoEps + b0));
+    double a1 = dr<D>(0.6, (kWenoEps + b1) * (kWenoEps + b1));
+    double a2 = dr<D>(0.3, (kWenoEps + b2) * (kWenoEps + b2));
     double s = a0 + a1 + a2;
-    const double ys = 1.0 / s;
+    const double ys = D == kDivIEEE ? 0.0 : dr<D>(1.0, s);
     double w0 = dq<D>(a0, s, ys), w1 = dq<D>(a1, s, ys), w2 = dq<D>(a2, s, ys);
     double p0 = d6<D>(2.0 * v0 - 7.0 * v1 + 11.0 * v2);
     double p1 = d6<D>(-v1 + 5.0 * v2 + 2.0 * v3);
@@ -70,11 +70,11 @@ __device__ __forceinline__ double weno_r(double v0, double v1, double v2, double
     double b0 = beta3(v4, v3, v2);
     double b1 = beta3(v3, v2, v1);
     double b2 = beta3(v2, v1, v0);
-    double a0 = 0.3 / ((kWenoEps + b0) * (kWenoEps + b0));
-    double a1 = 0.6 / ((kWenoEps + b1) * (kWenoEps + b1));
-    double a2 = 0.1 / ((kWenoEps + b2) * (kWenoEps + b2));
+    double a0 = dr<D>(0.3, (kWenoEps + b0) * (kWenoEps + b0));
+    double a1 = dr<D>(0.6, (kWenoEps + b1) * (kWenoEps + b1));
+    double a2 = dr<D>(0.1, (kWenoEps + b2) * (kWenoEps + b2));
     double s = a0 + a1 + a2;
-    const double ys = 1.0 / s;
+    const double ys = D == kDivIEEE ? 0.0 : dr<D>(1.0, s);
     double w0 = dq<D>(a0, s, ys), w1 = dq<D>(a1, s, ys), w2 = dq<D>(a2, s, ys);
     double p0 = d6<D>(11.0 * v2 - 7.0 * v3 + 2.0 * v4);
     double p1 = d6<D>(2.0 * v1 + 5.0 * v2 - v3);
@@ -95,11 +95,11 @@ __device__ __forceinline__ double weno_u(double u0, double u1, double u2, double
     double b1 = beta3(u1, u2, u3);
     double b2 = beta3(u2, u3, u4);
     const double d0 = mirror ? 0.3 : 0.1, d2 = mirror ? 0.1 : 0.3;
-    double a0 = d0 / ((kWenoEps + b0) * (kWenoEps + b0));
-    double a1 = 0.6 / ((kWenoEps + b1) * (kWenoEps + b1));
-    double a2 = d2 / ((kWenoEps + b2) * (kWenoEps + b2));
+    double a0 = dr<D>(d0, (kWenoEps + b0) * (kWenoEps + b0));
+    double a1 = dr<D>(0.6, (kWenoEps + b1) * (kWenoEps + b1));
+    double a2 = dr<D>(d2, (kWenoEps + b2) * (kWenoEps + b2));
     double s = a0 + a1 + a2;
-    const double ys = D == kDivIEEE ? 0.0 : 1.0 / s;
+    const double ys = D == kDivIEEE ? 0.0 : dr<D>(1.0, s);
     double w0 = dq<D>(a0, s, ys), w1 = dq<D>(a1, s, ys), w2 = dq<D>(a2, s, ys);
     // left: p0 = (2u0 - 7u1) + 11u2, p1 = (-u1 + 5u2) + 2u3, p2 = (2u2 + 5u3) - u4;
     // right: the first and last term of each swapped

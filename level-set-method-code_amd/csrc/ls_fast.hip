@@ -453,7 +453,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
                     gz = lsx::dq<D>(zG[4] - zG[2], 2.0 * z.dz, 0.5 * z.rdz);
                     gm = sqrt(gx * gx + gy * gy + gz * gz + z.eps);
                 }
-                const double ygm = D == lsx::kDivIEEE ? 0.0 : 1.0 / gm;
+                const double ygm = D == lsx::kDivIEEE ? 0.0 : lsx::dr<D>(1.0, gm);
                 ue = cu - lsx::dq<D>(csl * gx, gm, ygm);
                 ve = cv - lsx::dq<D>(csl * gy, gm, ygm);
                 we = cw - lsx::dq<D>(csl * gz, gm, ygm);

@@ -369,6 +369,21 @@ using lsi::make_geom;
 
 static hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// The exact mode's division sequences on the device, one quotient per thread:
+// kind 0 rdiv_u(a, b); kind 1 qdiv_u(a, b, y) with y = rdiv_u(1, b), as the
+// weights use it; kind 2 div6_u(a).
+__global__ void k_div_check(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out,
+                            int64_t n, int kind) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = a[i];
+    double q;
+    if (kind == 0) q = lsx::rdiv_u(x, b[i]);
+    else if (kind == 1) q = lsx::qdiv_u(x, b[i], lsx::rdiv_u(1.0, b[i]));
+    else q = lsx::div6_u(x);
+    out[i] = q;
+}
+
 static dim3 cell_grid(const Geom& g) {
     return dim3((g.nx + kTX - 1) / kTX, (g.ny + kTY - 1) / kTY, g.dim == 3 ? g.nz : 1);
 }
@@ -750,6 +765,17 @@ int ls_diag_noalloc(const ls_params_t* p, const double* G, const double* G0, dou
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, part, kRedBlocks);
     hipLaunchKernelGGL(k_diag_emit, dim3(1), dim3(64), 0, st, (const DiagPart*)(part + kRedBlocks), out,
                        (double)ncell, cell, G0 ? 1 : 0);
+    LS_HIP(hipGetLastError());
+    return LS_OK;
+}
+
+int ls_div_check(const double* a, const double* b, double* out, int64_t n, int32_t kind, void* stream) {
+    if (!a || !out || (kind != 2 && !b)) return fail(LS_ERR_INVALID, "NULL argument");
+    if (n < 0 || kind < 0 || kind > 2) return fail(LS_ERR_INVALID, "bad count or kind");
+    if (n == 0) return LS_OK;
+    const int64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffffLL) return fail(LS_ERR_INVALID, "count too large for one launch");
+    hipLaunchKernelGGL(k_div_check, dim3((unsigned)blocks), dim3(256), 0, S(stream), a, b, out, n, kind);
     LS_HIP(hipGetLastError());
     return LS_OK;
 }

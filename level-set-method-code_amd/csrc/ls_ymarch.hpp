@@ -266,7 +266,7 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
                     gy = lsx::dq<D>(r[4] - r[2], 2.0 * m.dy, 0.5 * m.rdy);
                     gm = sqrt(gx * gx + gy * gy + m.eps);
                 }
-                const double ygm = D == lsx::kDivIEEE ? 0.0 : 1.0 / gm;
+                const double ygm = D == lsx::kDivIEEE ? 0.0 : lsx::dr<D>(1.0, gm);
                 ue = cur.u - lsx::dq<D>(cur.sl * gx, gm, ygm);
                 ve = cur.v - lsx::dq<D>(cur.sl * gy, gm, ygm);
             };
