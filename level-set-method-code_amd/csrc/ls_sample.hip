@@ -62,11 +62,11 @@ __device__ __forceinline__ double sface(double v0, double v1, double v2, double 
     const double b0 = sbeta(v0, v1, v2);
     const double b1 = sbeta(v1, v2, v3);
     const double b2 = sbeta(v2, v3, v4);
-    const double a0 = 0.1 / ((kWenoEps + b0) * (kWenoEps + b0));
-    const double a1 = 0.6 / ((kWenoEps + b1) * (kWenoEps + b1));
-    const double a2 = 0.3 / ((kWenoEps + b2) * (kWenoEps + b2));
+    const double a0 = lsx::dr<D>(0.1, (kWenoEps + b0) * (kWenoEps + b0));
+    const double a1 = lsx::dr<D>(0.6, (kWenoEps + b1) * (kWenoEps + b1));
+    const double a2 = lsx::dr<D>(0.3, (kWenoEps + b2) * (kWenoEps + b2));
     const double sum = a0 + a1 + a2;
-    const double ys = D == lsx::kDivIEEE ? 0.0 : 1.0 / sum;
+    const double ys = D == lsx::kDivIEEE ? 0.0 : lsx::dr<D>(1.0, sum);
     const double w0 = lsx::dq<D>(a0, sum, ys), w1 = lsx::dq<D>(a1, sum, ys), w2 = lsx::dq<D>(a2, sum, ys);
     return lsx::d6<D>(w0 * (2.0 * v0 - 7.0 * v1 + 11.0 * v2)) + lsx::d6<D>(w1 * (-v1 + 5.0 * v2 + 2.0 * v3)) +
            lsx::d6<D>(w2 * (2.0 * v2 + 5.0 * v3 - v4));
@@ -86,7 +86,8 @@ __device__ __forceinline__ double srhs_f(const SGeom& g, double fmx, double f0x,
     const double propagation = S_L * sqrt(Gdx_mean * Gdx_mean + Gdy_mean * Gdy_mean);
     const double nm = sqrt(Gdxm * Gdxm + Gdym * Gdym + 1e-14);
     const double np = sqrt(Gdxp * Gdxp + Gdyp * Gdyp + 1e-14);
-    const double ynm = D == lsx::kDivIEEE ? 0.0 : 1.0 / nm, ynp = D == lsx::kDivIEEE ? 0.0 : 1.0 / np;
+    const double ynm = D == lsx::kDivIEEE ? 0.0 : lsx::dr<D>(1.0, nm);
+    const double ynp = D == lsx::kDivIEEE ? 0.0 : lsx::dr<D>(1.0, np);
     const double s_xm = uc - lsx::dq<D>(S_L * Gdxm, nm, ynm);
     const double s_xp = uc - lsx::dq<D>(S_L * Gdxp, np, ynp);
     const double alpha_x = dmax(fabs(s_xm), fabs(s_xp));
