@@ -24,8 +24,8 @@
 //
 // Each cell's partial derivatives P_q[c] = lambda_c * d out_c / d G[c + off_q]
 // must be summed at the target cells c + off_q.  The stage adjoint (k_adj_zm)
-// sums its x and z partials inside the block and leaves the y partials and
-// the block-edge spills to a combine kernel; the HCR-2 adjoint writes its
+// sums its partials inside the block and leaves the block-edge spills to a
+// combine kernel; the HCR-2 adjoint writes its
 // 7-point partials to planes that a gather kernel sums.  Fixed summation
 // order, no atomics: the gradient is deterministic.
 #include <hip/hip_runtime.h>
@@ -213,28 +213,32 @@ struct ZMGeom {
 
 __device__ __forceinline__ int split_at(int n, int parts, int q) { return (int)(((int64_t)n * q) / parts); }
 
-// Stage partials, summed in the block along x and z.  A block walks its z
-// chunk one plane at a time; each cell computes its 19 partials (stage_partials).
+// Stage partials, summed in the block.  A block walks its z chunk one plane
+// at a time; each cell computes its 19 partials (stage_partials).
 //   * x: the 6 x partials of a row go through LDS; every cell of the segment
 //     sums the ones that land on it, and lanes 0..5 sum the ones that leave
 //     the segment (3 cells each side) into the spill plane EX.
+//   * y: likewise through LDS across the tile's rows; the sums for the 3 rows
+//     below and the 3 above the tile go to the spill planes EYlo and EYhi.
+//     (Those of two neighbouring 4-row tiles overlap, hence two planes.)
 //   * z: each cell carries accumulators for the 7 target planes k-3..k+3 of
 //     its column; a target plane is complete three planes later and goes to
 //     `core`, or, below and above the chunk, to the spill plane EZ.
-//   * y: the 6 y partials go to planes Py[0..5] as P_q[c] (gathered later).
 // A spill position has one writer per pass, because segments and chunks span
-// at least 6 cells, and every pass writes the same positions; positions never
-// written stay zero.  The sums run in a fixed order: the result is
-// deterministic.
+// at least 6 cells and tiles at least 3 rows, and every pass writes the same
+// positions; positions never written stay zero.  The sums run in a fixed
+// order: the result is deterministic.
 __global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_adj_zm(AGeom g, ZMGeom zm, const double* __restrict__ Gs,
                                                      const double* __restrict__ u, const double* __restrict__ v,
                                                      const double* __restrict__ w, const double* __restrict__ sl,
                                                      double sl0, const double* __restrict__ lam_out, double cdt,
-                                                     double* __restrict__ core, double* __restrict__ Py,
-                                                     double* __restrict__ EX, double* __restrict__ EZ,
+                                                     double* __restrict__ core, double* __restrict__ EX,
+                                                     double* __restrict__ EYlo, double* __restrict__ EYhi,
+                                                     double* __restrict__ EZ,
                                                      double* __restrict__ gu, double* __restrict__ gv,
                                                      double* __restrict__ gw, double* __restrict__ gsl) {
     __shared__ double sX[kAY][6][kAX];
+    __shared__ double sY[kAY][6][kAX];
     const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
     int b = (int)blockIdx.x;
     const int sxi = b % zm.nsx;
@@ -264,7 +268,7 @@ __global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8)
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 sX[ly][q][lx] = out[1 + q];
-                Py[(int64_t)q * g.n + c] = out[7 + q];
+                sY[ly][q][lx] = out[7 + q];
             }
             // z partials: out[13 + q] lands on plane k + o, o = -3,-2,-1,1,2,3
             zacc[3] += out[0];
@@ -276,18 +280,49 @@ __global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8)
             zacc[6] += out[18];
         } else {
 #pragma unroll
-            for (int q = 0; q < 6; q++) sX[ly][q][lx] = 0.0;
+            for (int q = 0; q < 6; q++) {
+                sX[ly][q][lx] = 0.0;
+                sY[ly][q][lx] = 0.0;
+            }
         }
         __syncthreads();
-        // x partial q of source s lands on s + o, o = -3,-2,-1,1,2,3
+        // x (y) partial q of source s lands on s + o, o = -3,-2,-1,1,2,3
         if (act) {
-            double xsum = 0.0;
+            double xsum = 0.0, ysum = 0.0;
 #pragma unroll
             for (int q = 0; q < 6; q++) {
-                const int s = lx - (q < 3 ? q - 3 : q - 2);
+                const int o = q < 3 ? q - 3 : q - 2;
+                const int s = lx - o, r = ly - o;
                 if (s >= 0 && s < L) xsum += sX[ly][q][s];
+                if (r >= 0 && r < kAY) ysum += sY[r][q][lx];
             }
-            zacc[3] += xsum;
+            zacc[3] += xsum + ysum;
+        } else if (lx < L && j <= g.ny + 2) {
+            // a ghost row inside the tile (j >= ny): only the y partials of the
+            // tile's own rows land here; no other writer of core at this position
+            double ysum = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const int r = ly - (q < 3 ? q - 3 : q - 2);
+                if (r >= 0 && r < kAY) ysum += sY[r][q][lx];
+            }
+            core[cell(g, i, j, k)] = ysum;
+        }
+        // the 3 rows below (EYlo) and above (EYhi) the tile, column lx
+        if (lx < L) {
+            for (int e = ly; e < 6; e += kAY) {
+                const int r = e < 3 ? e - 3 : kAY + e - 3;
+                const int jt = ty * kAY + r;
+                if (jt <= g.ny + 2) {
+                    double ye = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 6; q++) {
+                        const int s = r - (q < 3 ? q - 3 : q - 2);
+                        if (s >= 0 && s < kAY) ye += sY[s][q][lx];
+                    }
+                    (e < 3 ? EYlo : EYhi)[cell(g, i, jt, k)] = ye;
+                }
+            }
         }
         if (row && lx < 6) {  // the 3 targets on each side of the segment
             const int p = lx < 3 ? lx - 3 : L + lx - 3;
@@ -317,22 +352,17 @@ __global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8)
     }
 }
 
-// target[p] = a * self[p] + core[p] + EX[p] + EZ[p] + sum_q Py_q[p - off_q]
-// (+ target[p] if accumulate) over every padded cell p: the stage adjoint's
-// y gather plus the block sums of k_adj_zm.  Planes are zero where nothing
-// was written, so a source index that leaves the array contributes nothing.
+// target[p] = a * self[p] + core[p] + EX[p] + EYlo[p] + EYhi[p] + EZ[p]
+// (+ target[p] if accumulate) over every padded cell p: the block sums of
+// k_adj_zm and its spills.  Planes are zero where nothing was written.
 __global__ __launch_bounds__(256) void k_adj_combine(AGeom g, const double* __restrict__ core,
-                                                     const double* __restrict__ Py, const double* __restrict__ EX,
-                                                     const double* __restrict__ EZ, const double* __restrict__ self,
-                                                     double a, double* __restrict__ target, int accumulate) {
+                                                     const double* __restrict__ EX, const double* __restrict__ EYlo,
+                                                     const double* __restrict__ EYhi, const double* __restrict__ EZ,
+                                                     const double* __restrict__ self, double a,
+                                                     double* __restrict__ target, int accumulate) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n; p += (int64_t)gridDim.x * blockDim.x) {
         double acc = self ? a * self[p] : 0.0;
-        acc += core[p] + EX[p] + EZ[p];
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            const int64_t src = p - (int64_t)(q < 3 ? q - 3 : q - 2) * g.sy;
-            if (src >= 0 && src < g.n) acc += Py[(int64_t)q * g.n + src];
-        }
+        acc += core[p] + EX[p] + EYlo[p] + EYhi[p] + EZ[p];
         target[p] = accumulate ? target[p] + acc : acc;
     }
 }
@@ -559,7 +589,7 @@ struct ls_adjoint {
     int nsteps = 0, ckpt = 0, nseg = 0, iters = 0;
     size_t fb = 0;
     ZMGeom zm;                  // block layout of the z-marching stage adjoint
-    double* P = nullptr;        // 19 partial planes (ghost regions stay zero; 14..16 are the stage adjoint's)
+    double* P = nullptr;        // 19 partial planes (ghost regions stay zero; 0..13 HCR-2's, 14..18 the stage adjoint's)
     double* lam = nullptr;      // adjoint of the current state
     double* lam2 = nullptr;     // scratch adjoints
     double* lam3 = nullptr;
@@ -713,13 +743,15 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
     const double sl0 = a->p.s_l;
     const dim3 cgrid((g.nx + kAX - 1) / kAX, (g.ny + kAY - 1) / kAY, g.nz), cblk(kAX, kAY, 1);
     const dim3 lgrid(1024), lblk(256);
-    // z-marching stage adjoint: its y planes are P[0..5], its block sums and
-    // spills P[14..16] (planes nothing else writes)
+    // z-marching stage adjoint: its block sums and spills are P[14..18]
+    // (planes nothing else writes)
     const ZMGeom zm = a->zm;
     const dim3 zgrid((unsigned)(zm.nsx * zm.nty * zm.nch));
     double* core = a->P + 14 * g.n;
     double* EX = a->P + 15 * g.n;
     double* EZ = a->P + 16 * g.n;
+    double* EYlo = a->P + 17 * g.n;
+    double* EYhi = a->P + 18 * g.n;
     int rc = LS_OK;
 #define ADJ_HIP(call)                                                          \
     do {                                                                       \
@@ -785,11 +817,12 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
         const double C = stage == 1 ? 1.0 : (stage == 2 ? 0.25 : 2.0 / 3.0);
         hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, lam_in);
         hipLaunchKernelGGL(k_adj_zm, zgrid, cblk, 0, st, g, zm, Gs, u, v, w, sl_field, sl0, lam_in, C * dt, core,
-                           a->P, EX, EZ, gu, gv, gw, gsl);
+                           EX, EYlo, EYhi, EZ, gu, gv, gw, gsl);
         if (stage == 1) {
-            hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, a->P, EX, EZ, lam_in, A + B, a->acc, 1);
+            hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, EX, EYlo, EYhi, EZ, lam_in, A + B, a->acc,
+                               1);
         } else {
-            hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, a->P, EX, EZ, lam_in, B, lam_s, 0);
+            hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, EX, EYlo, EYhi, EZ, lam_in, B, lam_s, 0);
             hipLaunchKernelGGL(k_axpy, lgrid, lblk, 0, st, g.n, A, lam_in, a->acc);
         }
         ADJ_HIP(hipGetLastError());
