@@ -149,61 +149,6 @@ __device__ __forceinline__ int64_t off7(const AGeom& g, int q) {
     }
 }
 
-// Stage partials of one cell from its pointwise inputs (uc, vc, wc, s_l) and
-// lam = cdt * lambda_out[c] (cdt = the stage's coefficient of dt*RHS times
-// dt): out[q] = lam * dRHS_c/dG[c + o_q] with q = 0 the centre, 1..6 the x
-// offsets -3,-2,-1,+1,+2,+3, 7..12 y, 13..18 z.  Also returns the derivative
-// (dgx, dgy, dgz) and, when S_L is active, the S_L partial qim = Q/m (the
-// pointwise adjoints are -lam dg and lam qim).
-__device__ __forceinline__ bool stage_partials(const AGeom& g, const double* __restrict__ Gs, int64_t c, double uc,
-                                               double vc, double wc, double s_l, double lam, double (&out)[19],
-                                               double& dgx, double& dgy, double& dgz, double& qim) {
-    const double* G = Gs + c;
-    const double ihx = 1.0 / g.dx, ihy = 1.0 / g.dy, ihz = 1.0 / g.dz;
-    double gx = 0.0, gy = 0.0, gz = 0.0, m = 1.0, im = 1.0;
-    const bool on = s_l > g.eps;
-    if (on) {  // u_eff in the exact forward's arithmetic: its sign picks the upwind side
-        gx = (G[1] - G[-1]) / (2.0 * g.dx);
-        gy = (G[g.sy] - G[-g.sy]) / (2.0 * g.dy);
-        gz = (G[g.sz] - G[-g.sz]) / (2.0 * g.dz);
-        m = sqrt(gx * gx + gy * gy + gz * gz + g.eps);
-        im = lsf::rcp_nr(m);
-    }
-    const double ue = uc - s_l * gx / m, ve = vc - s_l * gy / m, we = wc - s_l * gz / m;
-    double s7[7], Px[7], Py[7], Pz[7];
-    for (int q = 0; q < 7; q++) s7[q] = G[q - 3];
-    dgx = dweno_grad(s7, ue, ihx, Px);
-    for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sy];
-    dgy = dweno_grad(s7, ve, ihy, Py);
-    for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sz];
-    dgz = dweno_grad(s7, we, ihz, Pz);
-    out[0] = -(ue * Px[3] + ve * Py[3] + we * Pz[3]);
-    for (int q = 0; q < 6; q++) {
-        const int t = q < 3 ? q : q + 1;
-        out[1 + q] = -ue * Px[t];
-        out[7 + q] = -ve * Py[t];
-        out[13 + q] = -we * Pz[t];
-    }
-    qim = 0.0;
-    if (on) {
-        // d RHS / d g_b = s_l (dg_b - g_b Q / m^2) / m with Q = sum_a dg_a g_a
-        const double Q = dgx * gx + dgy * gy + dgz * gz;
-        const double qm = Q * (im * im);
-        const double rx = s_l * (dgx - gx * qm) * im;
-        const double ry = s_l * (dgy - gy * qm) * im;
-        const double rz = s_l * (dgz - gz * qm) * im;
-        out[3] -= rx * (0.5 * ihx);
-        out[4] += rx * (0.5 * ihx);
-        out[9] -= ry * (0.5 * ihy);
-        out[10] += ry * (0.5 * ihy);
-        out[15] -= rz * (0.5 * ihz);
-        out[16] += rz * (0.5 * ihz);
-        qim = Q * im;
-    }
-    for (int q = 0; q < 19; q++) out[q] *= lam;
-    return on;
-}
-
 // Blocks of the z-marching stage adjoint: x segments of at most 64 cells
 // (one wavefront row each), 4-row y tiles, z chunks; segments and chunks are
 // balanced, so every segment and chunk spans at least 6 cells.
@@ -214,7 +159,9 @@ struct ZMGeom {
 __device__ __forceinline__ int split_at(int n, int parts, int q) { return (int)(((int64_t)n * q) / parts); }
 
 // Stage partials, summed in the block.  A block walks its z chunk one plane
-// at a time; each cell computes its 19 partials (stage_partials).
+// at a time; each cell c computes its 19 partials lam_c * dRHS_c/dG[c + o]
+// (lam_c = cdt * lam_out[c]): the centre and the offsets -3..3 of each axis,
+// through the WENO-5 faces (dweno_grad) and the central gradient of u_eff.
 //   * x: the 6 x partials of a row go through LDS; every cell of the segment
 //     sums the ones that land on it, and lanes 0..5 sum the ones that leave
 //     the segment (3 cells each side) into the spill plane EX.
@@ -259,25 +206,75 @@ __global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8)
             const double lam = cdt * lam_out[c];
             const double uc = u[c], vc = v[c], wc = w[c], s_l = sl ? sl[c] : sl0;
             const double gu0 = gu[c], gv0 = gv[c], gw0 = gw[c], gs0 = gsl ? gsl[c] : 0.0;
-            double out[19], dgx, dgy, dgz, qim;
-            const bool on = stage_partials(g, Gs, c, uc, vc, wc, s_l, lam, out, dgx, dgy, dgz, qim);
+            // The partials one axis at a time, each stored as soon as it is
+            // known, so that the 19 are never live together: x and y into LDS,
+            // z into the accumulators.
+            const double* G = Gs + c;
+            const double ihx = 1.0 / g.dx, ihy = 1.0 / g.dy, ihz = 1.0 / g.dz;
+            double gx = 0.0, gy = 0.0, gz = 0.0, m = 1.0, im = 1.0;
+            const bool on = s_l > g.eps;
+            if (on) {  // u_eff in the exact forward's arithmetic: its sign picks the upwind side
+                gx = (G[1] - G[-1]) / (2.0 * g.dx);
+                gy = (G[g.sy] - G[-g.sy]) / (2.0 * g.dy);
+                gz = (G[g.sz] - G[-g.sz]) / (2.0 * g.dz);
+                m = sqrt(gx * gx + gy * gy + gz * gz + g.eps);
+                im = lsf::rcp_nr(m);
+            }
+            const double ue = uc - s_l * gx / m, ve = vc - s_l * gy / m, we = wc - s_l * gz / m;
+            double s7[7], P[7];
+#pragma unroll
+            for (int q = 0; q < 7; q++) s7[q] = G[q - 3];
+            const double dgx = dweno_grad(s7, ue, ihx, P);
+            double ctr = ue * P[3];
+            double xm1 = -ue * P[2], xp1 = -ue * P[4];  // the offsets -1, +1 get the S_L terms too
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                if (q != 2 && q != 3) sX[ly][q][lx] = (-ue * P[q < 3 ? q : q + 1]) * lam;
+#pragma unroll
+            for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sy];
+            const double dgy = dweno_grad(s7, ve, ihy, P);
+            ctr += ve * P[3];
+            double ym1 = -ve * P[2], yp1 = -ve * P[4];
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                if (q != 2 && q != 3) sY[ly][q][lx] = (-ve * P[q < 3 ? q : q + 1]) * lam;
+#pragma unroll
+            for (int q = 0; q < 7; q++) s7[q] = G[(int64_t)(q - 3) * g.sz];
+            const double dgz = dweno_grad(s7, we, ihz, P);
+            ctr += we * P[3];
+            double zm1 = -we * P[2], zp1 = -we * P[4];
+            // z partial to plane k + o, o = -3, -2, +2, +3 (the +-1 ones below)
+            zacc[0] += (-we * P[0]) * lam;
+            zacc[1] += (-we * P[1]) * lam;
+            zacc[5] += (-we * P[5]) * lam;
+            zacc[6] += (-we * P[6]) * lam;
+            double qim = 0.0;
+            if (on) {
+                // d RHS / d g_b = s_l (dg_b - g_b Q / m^2) / m with Q = sum_a dg_a g_a
+                const double Q = dgx * gx + dgy * gy + dgz * gz;
+                const double qm = Q * (im * im);
+                const double rx = s_l * (dgx - gx * qm) * im;
+                const double ry = s_l * (dgy - gy * qm) * im;
+                const double rz = s_l * (dgz - gz * qm) * im;
+                xm1 -= rx * (0.5 * ihx);
+                xp1 += rx * (0.5 * ihx);
+                ym1 -= ry * (0.5 * ihy);
+                yp1 += ry * (0.5 * ihy);
+                zm1 -= rz * (0.5 * ihz);
+                zp1 += rz * (0.5 * ihz);
+                qim = Q * im;
+            }
+            sX[ly][2][lx] = xm1 * lam;
+            sX[ly][3][lx] = xp1 * lam;
+            sY[ly][2][lx] = ym1 * lam;
+            sY[ly][3][lx] = yp1 * lam;
+            zacc[2] += zm1 * lam;
+            zacc[4] += zp1 * lam;
+            zacc[3] += -ctr * lam;
             gu[c] = gu0 - lam * dgx;
             gv[c] = gv0 - lam * dgy;
             gw[c] = gw0 - lam * dgz;
             if (gsl && on) gsl[c] = gs0 + lam * qim;
-#pragma unroll
-            for (int q = 0; q < 6; q++) {
-                sX[ly][q][lx] = out[1 + q];
-                sY[ly][q][lx] = out[7 + q];
-            }
-            // z partials: out[13 + q] lands on plane k + o, o = -3,-2,-1,1,2,3
-            zacc[3] += out[0];
-            zacc[0] += out[13];
-            zacc[1] += out[14];
-            zacc[2] += out[15];
-            zacc[4] += out[16];
-            zacc[5] += out[17];
-            zacc[6] += out[18];
         } else {
 #pragma unroll
             for (int q = 0; q < 6; q++) {
