@@ -351,16 +351,20 @@ __global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8)
 
 // target[p] = a * self[p] + core[p] + EX[p] + EYlo[p] + EYhi[p] + EZ[p]
 // (+ target[p] if accumulate) over every padded cell p: the block sums of
-// k_adj_zm and its spills.  Planes are zero where nothing was written.
+// k_adj_zm and its spills.  Planes are zero where nothing was written.  With
+// acc2 given, also acc2[p] += a2 * self[p] (the stage's G_n term).
 __global__ __launch_bounds__(256) void k_adj_combine(AGeom g, const double* __restrict__ core,
                                                      const double* __restrict__ EX, const double* __restrict__ EYlo,
                                                      const double* __restrict__ EYhi, const double* __restrict__ EZ,
                                                      const double* __restrict__ self, double a,
-                                                     double* __restrict__ target, int accumulate) {
+                                                     double* __restrict__ target, int accumulate,
+                                                     double* __restrict__ acc2, double a2) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n; p += (int64_t)gridDim.x * blockDim.x) {
-        double acc = self ? a * self[p] : 0.0;
+        const double sp = self ? self[p] : 0.0;
+        double acc = self ? a * sp : 0.0;
         acc += core[p] + EX[p] + EYlo[p] + EYhi[p] + EZ[p];
         target[p] = accumulate ? target[p] + acc : acc;
+        if (acc2) acc2[p] += a2 * sp;
     }
 }
 
@@ -815,13 +819,12 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
         hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, lam_in);
         hipLaunchKernelGGL(k_adj_zm, zgrid, cblk, 0, st, g, zm, Gs, u, v, w, sl_field, sl0, lam_in, C * dt, core,
                            EX, EYlo, EYhi, EZ, gu, gv, gw, gsl);
-        if (stage == 1) {
+        if (stage == 1)
             hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, EX, EYlo, EYhi, EZ, lam_in, A + B, a->acc,
-                               1);
-        } else {
-            hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, EX, EYlo, EYhi, EZ, lam_in, B, lam_s, 0);
-            hipLaunchKernelGGL(k_axpy, lgrid, lblk, 0, st, g.n, A, lam_in, a->acc);
-        }
+                               1, (double*)nullptr, 0.0);
+        else
+            hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, EX, EYlo, EYhi, EZ, lam_in, B, lam_s, 0,
+                               a->acc, A);
         ADJ_HIP(hipGetLastError());
         return LS_OK;
     };
