@@ -397,11 +397,48 @@ __device__ __forceinline__ int img_at(int q, int n, int bc, int a) {
     return q == 0 ? a - 1 : n + 2 + a;
 }
 
+// Cells within 3 of a face of the interior (the only ones with ghost images),
+// enumerated as the z slabs (6 planes), then the y slabs of the remaining
+// planes, then the x slabs of the remaining rows.  Needs nx, ny, nz >= 6.
+__host__ __device__ __forceinline__ int64_t shell_cells(int nx, int ny, int nz) {
+    return 6LL * nx * ny + 6LL * nx * (nz - 6) + 6LL * (ny - 6) * (nz - 6);
+}
+__device__ __forceinline__ bool shell_cell(const AGeom& g, int64_t e, int& i, int& j, int& k) {
+    const int64_t a = 6LL * g.nx * g.ny, b = 6LL * g.nx * (g.nz - 6), c = 6LL * (g.ny - 6) * (g.nz - 6);
+    if (e < a) {
+        const int64_t pl = (int64_t)g.nx * g.ny;
+        const int kk = (int)(e / pl), r = (int)(e - kk * pl);
+        k = kk < 3 ? kk : g.nz - 6 + kk;
+        j = r / g.nx;
+        i = r - j * g.nx;
+        return true;
+    }
+    e -= a;
+    if (e < b) {
+        const int64_t pl = 6LL * g.nx;
+        const int kz = (int)(e / pl), r = (int)(e - kz * pl), jj = r / g.nx;
+        k = 3 + kz;
+        j = jj < 3 ? jj : g.ny - 6 + jj;
+        i = r - jj * g.nx;
+        return true;
+    }
+    e -= b;
+    if (e < c) {
+        const int64_t pl = 6LL * (g.ny - 6);
+        const int kz = (int)(e / pl), r = (int)(e - kz * pl), ii = r % 6;
+        k = 3 + kz;
+        j = 3 + r / 6;
+        i = ii < 3 ? ii : g.nx - 6 + ii;
+        return true;
+    }
+    return false;
+}
+
 // Adjoint of a ghost fill: each interior cell near a face adds the adjoints of
-// the ghost cells that copy it and clears them.
+// the ghost cells that copy it and clears them.  One thread per shell cell.
 __global__ __launch_bounds__(256) void k_fold(AGeom g, double* __restrict__ lam) {
-    const int i = blockIdx.x * kAX + threadIdx.x, j = blockIdx.y * kAY + threadIdx.y, k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
+    int i, j, k;
+    if (!shell_cell(g, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const int ni = n_img(i, g.nx, g.bc[0]), nj = n_img(j, g.ny, g.bc[1]), nk = n_img(k, g.nz, g.bc[2]);
     if (ni * nj * nk == 1) return;
     double acc = 0.0;
@@ -744,6 +781,7 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
     const double sl0 = a->p.s_l;
     const dim3 cgrid((g.nx + kAX - 1) / kAX, (g.ny + kAY - 1) / kAY, g.nz), cblk(kAX, kAY, 1);
     const dim3 lgrid(1024), lblk(256);
+    const dim3 fgrid((unsigned)((shell_cells(g.nx, g.ny, g.nz) + 255) / 256));
     // z-marching stage adjoint: its block sums and spills are P[14..18]
     // (planes nothing else writes)
     const ZMGeom zm = a->zm;
@@ -816,7 +854,7 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
         const double A = stage == 1 ? 1.0 : (stage == 2 ? 0.75 : 1.0 / 3.0);
         const double B = stage == 1 ? 0.0 : (stage == 2 ? 0.25 : 2.0 / 3.0);
         const double C = stage == 1 ? 1.0 : (stage == 2 ? 0.25 : 2.0 / 3.0);
-        hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, lam_in);
+        hipLaunchKernelGGL(k_fold, fgrid, lblk, 0, st, g, lam_in);
         hipLaunchKernelGGL(k_adj_zm, zgrid, cblk, 0, st, g, zm, Gs, u, v, w, sl_field, sl0, lam_in, C * dt, core,
                            EX, EYlo, EYhi, EZ, gu, gv, gw, gsl);
         if (stage == 1)
@@ -835,7 +873,7 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
         ADJ_HIP(hipMemsetAsync(a->lam2, 0, fb, st));  // adjoint of phi_0 through the r~ and sign terms
         for (int it = iters - 1; it >= 0; it--) {
             const double* in = it == 0 ? phi0 : a->hit[it];
-            hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, a->lam);
+            hipLaunchKernelGGL(k_fold, fgrid, lblk, 0, st, g, a->lam);
             hipLaunchKernelGGL(k_adj_hcr2, cgrid, cblk, 0, st, g, in, phi0, a->lam, a->P, a->P + 7 * g.n);
             hipLaunchKernelGGL(k_gather7, lgrid, lblk, 0, st, g, a->P + 7 * g.n, (const double*)nullptr, 0.0,
                                a->lam2, 1);
@@ -876,7 +914,7 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
         }
     }
     // G0's ghosts were filled from its interior before the first step.
-    hipLaunchKernelGGL(k_fold, cgrid, cblk, 0, st, g, a->lam);
+    hipLaunchKernelGGL(k_fold, fgrid, lblk, 0, st, g, a->lam);
     ADJ_HIP(hipGetLastError());
     if (gG0) ADJ_HIP(hipMemcpyAsync(gG0, a->lam, fb, hipMemcpyDeviceToDevice, st));
     ADJ_HIP(hipStreamSynchronize(st));
