@@ -46,6 +46,9 @@ CASES = [
     ("zero_grad_scalar_sl", (16, 17, 18), (1, 1, 1), False, 5, 0, 2, 0, 1e-10),
     ("mixed_bc_reinit", (18, 16, 20), (0, 1, 0), True, 6, 3, 6, 0, 1e-10),
     ("fast_forward", (20, 18, 16), (0, 0, 0), True, 7, 2, 3, 1, 1e-8),
+    # two x segments of the stage adjoint (spills between them), ny % 4 == 3
+    # (ghost rows inside the last 4-row tile), two z chunks
+    ("two_segments_reinit", (70, 19, 13), (0, 1, 0), True, 5, 2, 3, 0, 1e-10),
 ]
 
 
