@@ -60,8 +60,14 @@ __device__ __forceinline__ int64_t cell(const AGeom& g, int i, int j, int k) {
     return (int64_t)(k + 3) * g.sz + (int64_t)(j + 3) * g.sy + (i + 3);
 }
 
+// The face gradients below are derivatives held to the tangent checker's
+// tolerance, not to the forward's bits, so they let the compiler contract
+// a*b + c into FMAs (the library is built with contraction off).  The upwind
+// choice, which must match the forward, is made outside them.
+
 // Smoothness indicator of (a,b,c) and its gradient.
 __device__ __forceinline__ double beta_g(double a, double b, double c, double& da, double& db, double& dc) {
+#pragma clang fp contract(fast)
     const double D = a - 2.0 * b + c, E = c - a;
     da = (13.0 / 6.0) * D - 0.5 * E;
     db = -(13.0 / 3.0) * D;
@@ -76,6 +82,7 @@ __device__ __forceinline__ double beta_g(double a, double b, double c, double& d
 // the adjoint is a derivative, held to the tangent checker's tolerance, not to
 // the forward's bits.
 __device__ __forceinline__ double face_grad(const double* v, double d0, double d1, double d2, double* df) {
+#pragma clang fp contract(fast)
     double gb[3][3], b[3];
     b[0] = beta_g(v[0], v[1], v[2], gb[0][0], gb[0][1], gb[0][2]);
     b[1] = beta_g(v[1], v[2], v[3], gb[1][0], gb[1][1], gb[1][2]);
@@ -114,6 +121,7 @@ __device__ __forceinline__ double face_grad(const double* v, double d0, double d
 // Upwind WENO-5 derivative along one axis from the stencil s7 = (m3..p3), and
 // its gradient P[0..6]; ih = 1/h.
 __device__ __forceinline__ double dweno_grad(const double* s7, double vel, double ih, double* P) {
+#pragma clang fp contract(fast)
     double fa, fb, ga[5], gb[5];
     for (int q = 0; q < 7; q++) P[q] = 0.0;
     if (vel >= 0.0) {
