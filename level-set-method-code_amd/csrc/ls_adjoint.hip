@@ -24,9 +24,8 @@
 //
 // Each cell's partial derivatives P_q[c] = lambda_c * d out_c / d G[c + off_q]
 // must be summed at the target cells c + off_q.  The stage adjoint (k_adj_zm)
-// sums its partials inside the block and leaves the block-edge spills to a
-// combine kernel; the HCR-2 adjoint writes its
-// 7-point partials to planes that a gather kernel sums.  Fixed summation
+// and the HCR-2 adjoint (k_adj_hcr2_zm) sum their partials inside the block
+// and leave the block-edge spills to a combine kernel.  Fixed summation
 // order, no atomics: the gradient is deterministic.
 #include <hip/hip_runtime.h>
 
@@ -376,21 +375,6 @@ __global__ __launch_bounds__(256) void k_adj_combine(AGeom g, const double* __re
     }
 }
 
-// target[p] = a * self[p] + sum_q P_q[p - off7(q)] (+ target[p] if accumulate)
-// over every padded cell p.  The planes are zero outside the interior, so a
-// source index that leaves the array or wraps a row contributes nothing.
-__global__ __launch_bounds__(256) void k_gather7(AGeom g, const double* __restrict__ P, const double* __restrict__ self,
-                                                 double a, double* __restrict__ target, int accumulate) {
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n; p += (int64_t)gridDim.x * blockDim.x) {
-        double acc = self ? a * self[p] : 0.0;
-        for (int q = 0; q < 7; q++) {
-            const int64_t src = p - off7(g, q);
-            if (src >= 0 && src < g.n) acc += P[(int64_t)q * g.n + src];
-        }
-        target[p] = accumulate ? target[p] + acc : acc;
-    }
-}
-
 // Number of padded positions that hold interior index q along an axis of n
 // points (itself plus its ghost images), and the a-th of them.  Periodic:
 // padded q+3 and, for q < 3, q+3+n, for q >= n-3, q+3-n.  Zero gradient:
@@ -463,18 +447,12 @@ __global__ __launch_bounds__(256) void k_fold(AGeom g, double* __restrict__ lam)
 }
 
 // HCR-2 partials of one pseudo-time step phi_new = step(phi_n; phi_0):
-// Pn_q[c] = lam_c * d phi_new_c / d phi_n[c + off7(q)], P0_q likewise for phi_0.
-__global__ __launch_bounds__(256) void k_adj_hcr2(AGeom g, const double* __restrict__ pn, const double* __restrict__ p0,
-                                                  const double* __restrict__ lam_new, double* __restrict__ Pn,
-                                                  double* __restrict__ P0) {
-    const int i = blockIdx.x * kAX + threadIdx.x, j = blockIdx.y * kAY + threadIdx.y, k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
-    const int64_t c = cell(g, i, j, k);
-    const double lam = lam_new[c];
-    double N[7], Z[7], dN[7], dZ[7];
+// d phi_new_c / d phi_n[c + off7(q)] (dN) and / d phi_0[c + off7(q)] (dZ) of
+// one HCR-2 iteration at cell c, from phi_n (N) and phi_0 (Z) on its 7-point
+// stencil in off7 order.
+__device__ __forceinline__ void hcr2_partials(const AGeom& g, const double (&N)[7], const double (&Z)[7],
+                                              double (&dN)[7], double (&dZ)[7]) {
     for (int q = 0; q < 7; q++) {
-        N[q] = pn[c + off7(g, q)];
-        Z[q] = p0[c + off7(g, q)];
         dN[q] = 0.0;
         dZ[q] = 0.0;
     }
@@ -550,9 +528,124 @@ __global__ __launch_bounds__(256) void k_adj_hcr2(AGeom g, const double* __restr
             }
         }
     }
-    for (int q = 0; q < 7; q++) {
-        Pn[(int64_t)q * g.n + c] = lam * dN[q];
-        P0[(int64_t)q * g.n + c] = lam * dZ[q];
+}
+
+// One HCR-2 iteration's adjoint, summed in the block like k_adj_zm.  The
+// partials lam_c * d phi_new_c / d phi_n[c + o] (and / d phi_0) land on the
+// cell and its six neighbours: x and y neighbours are summed through LDS, z
+// in three accumulators per column (targets k-1, k, k+1).  Spills: the column
+// left and right of the segment (EX), the row below and above the tile (EY),
+// the plane below and above the chunk (EZ); the spills of two neighbouring
+// blocks never coincide, so each plane has one writer per position.  The
+// sums for phi_n go to planes[0..3] (core, EX, EY, EZ), those for phi_0 to
+// planes[4..7]; positions never written stay zero.
+__global__ __launch_bounds__(kAX * kAY) void k_adj_hcr2_zm(AGeom g, ZMGeom zm, const double* __restrict__ pn,
+                                                            const double* __restrict__ p0,
+                                                            const double* __restrict__ lam_new,
+                                                            double* __restrict__ planes) {
+    __shared__ double sX[2][kAY][2][kAX];  // [phi_n, phi_0][row][to x-1, to x+1][lane]
+    __shared__ double sY[2][kAY][2][kAX];  // [phi_n, phi_0][row][to y-1, to y+1][lane]
+    const int lx = (int)threadIdx.x, ly = (int)threadIdx.y;
+    int b = (int)blockIdx.x;
+    const int sxi = b % zm.nsx;
+    b /= zm.nsx;
+    const int ty = b % zm.nty, ch = b / zm.nty;
+    const int xs = split_at(g.nx, zm.nsx, sxi), L = split_at(g.nx, zm.nsx, sxi + 1) - xs;
+    const int kb = split_at(g.nz, zm.nch, ch), ke = split_at(g.nz, zm.nch, ch + 1);
+    const int y0 = ty * kAY, i = xs + lx, j = y0 + ly;
+    const bool row = j < g.ny;
+    const bool act = row && lx < L;
+    double za[2][3];  // per output: targets k-1, k, k+1 of the current plane k
+#pragma unroll
+    for (int o = 0; o < 2; o++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) za[o][q] = 0.0;
+    for (int k = kb; k < ke; k++) {
+        if (act) {
+            const int64_t c = cell(g, i, j, k);
+            const double lam = lam_new[c];
+            double N[7], Z[7], dN[7], dZ[7];
+#pragma unroll
+            for (int q = 0; q < 7; q++) {
+                N[q] = pn[c + off7(g, q)];
+                Z[q] = p0[c + off7(g, q)];
+            }
+            hcr2_partials(g, N, Z, dN, dZ);
+#pragma unroll
+            for (int o = 0; o < 2; o++) {
+                const double(&D)[7] = o ? dZ : dN;
+                sX[o][ly][0][lx] = lam * D[1];
+                sX[o][ly][1][lx] = lam * D[2];
+                sY[o][ly][0][lx] = lam * D[3];
+                sY[o][ly][1][lx] = lam * D[4];
+                za[o][0] += lam * D[5];
+                za[o][1] += lam * D[0];
+                za[o][2] += lam * D[6];
+            }
+        } else {
+#pragma unroll
+            for (int o = 0; o < 2; o++)
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    sX[o][ly][e][lx] = 0.0;
+                    sY[o][ly][e][lx] = 0.0;
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+            double* core = planes + (int64_t)(4 * o) * g.n;
+            double* EX = core + g.n;
+            double* EY = core + 2 * g.n;
+            double ys = 0.0;  // y neighbours' partials landing on this cell
+            if (ly + 1 < kAY) ys += sY[o][ly + 1][0][lx];
+            if (ly >= 1) ys += sY[o][ly - 1][1][lx];
+            if (act) {
+                double xs_ = 0.0;
+                if (lx + 1 < L) xs_ += sX[o][ly][0][lx + 1];
+                if (lx >= 1) xs_ += sX[o][ly][1][lx - 1];
+                za[o][1] += xs_ + ys;
+            } else if (lx < L && j <= g.ny + 2) {
+                core[cell(g, i, j, k)] = ys;  // a ghost row inside the tile: y partials only
+            }
+            if (row && lx < 2)  // the column left (lane 0) and right (lane 1) of the segment
+                EX[cell(g, lx == 0 ? xs - 1 : xs + L, j, k)] = lx == 0 ? sX[o][ly][0][0] : sX[o][ly][1][L - 1];
+            if (lx < L) {  // the row below (row 0) and above (row 1) the tile
+                if (ly == 0) EY[cell(g, i, y0 - 1, k)] = sY[o][0][0][lx];
+                else if (ly == 1 && y0 + kAY <= g.ny + 2) EY[cell(g, i, y0 + kAY, k)] = sY[o][kAY - 1][1][lx];
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int o = 0; o < 2; o++) {
+                double* core = planes + (int64_t)(4 * o) * g.n;
+                double* EZ = core + 3 * g.n;
+                (k - 1 >= kb ? core : EZ)[cell(g, i, j, k - 1)] = za[o][0];
+                za[o][0] = za[o][1];
+                za[o][1] = za[o][2];
+                za[o][2] = 0.0;
+            }
+        }
+    }
+    if (act) {  // targets ke-1 (complete) and ke (above the chunk)
+#pragma unroll
+        for (int o = 0; o < 2; o++) {
+            double* core = planes + (int64_t)(4 * o) * g.n;
+            core[cell(g, i, j, ke - 1)] = za[o][0];
+            (core + 3 * g.n)[cell(g, i, j, ke)] = za[o][1];
+        }
+    }
+}
+
+// lamN[p] = the phi_n sums (planes 0..3) and lamZ[p] += the phi_0 sums
+// (planes 4..7) over every padded cell p.
+__global__ __launch_bounds__(256) void k_hcr2_combine(AGeom g, const double* __restrict__ planes,
+                                                      double* __restrict__ lamN, double* __restrict__ lamZ) {
+    const int64_t n = g.n;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        lamN[p] = ((planes[p] + planes[n + p]) + planes[2 * n + p]) + planes[3 * n + p];
+        lamZ[p] += ((planes[4 * n + p] + planes[5 * n + p]) + planes[6 * n + p]) + planes[7 * n + p];
     }
 }
 
@@ -635,7 +728,7 @@ struct ls_adjoint {
     int nsteps = 0, ckpt = 0, nseg = 0, iters = 0;
     size_t fb = 0;
     ZMGeom zm;                  // block layout of the z-marching stage adjoint
-    double* P = nullptr;        // 19 partial planes (ghost regions stay zero; 0..13 HCR-2's, 14..18 the stage adjoint's)
+    double* P = nullptr;        // 19 partial planes, zero where unwritten (0..7 HCR-2's, 14..18 the stage adjoint's)
     double* lam = nullptr;      // adjoint of the current state
     double* lam2 = nullptr;     // scratch adjoints
     double* lam3 = nullptr;
@@ -787,7 +880,7 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
     const int iters = a->iters;
     const int ckpt = a->ckpt;
     const double sl0 = a->p.s_l;
-    const dim3 cgrid((g.nx + kAX - 1) / kAX, (g.ny + kAY - 1) / kAY, g.nz), cblk(kAX, kAY, 1);
+    const dim3 cblk(kAX, kAY, 1);
     const dim3 lgrid(1024), lblk(256);
     const dim3 fgrid((unsigned)((shell_cells(g.nx, g.ny, g.nz) + 255) / 256));
     // z-marching stage adjoint: its block sums and spills are P[14..18]
@@ -882,10 +975,8 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
         for (int it = iters - 1; it >= 0; it--) {
             const double* in = it == 0 ? phi0 : a->hit[it];
             hipLaunchKernelGGL(k_fold, fgrid, lblk, 0, st, g, a->lam);
-            hipLaunchKernelGGL(k_adj_hcr2, cgrid, cblk, 0, st, g, in, phi0, a->lam, a->P, a->P + 7 * g.n);
-            hipLaunchKernelGGL(k_gather7, lgrid, lblk, 0, st, g, a->P + 7 * g.n, (const double*)nullptr, 0.0,
-                               a->lam2, 1);
-            hipLaunchKernelGGL(k_gather7, lgrid, lblk, 0, st, g, a->P, (const double*)nullptr, 0.0, a->lam3, 0);
+            hipLaunchKernelGGL(k_adj_hcr2_zm, zgrid, cblk, 0, st, g, zm, in, phi0, a->lam, a->P);
+            hipLaunchKernelGGL(k_hcr2_combine, lgrid, lblk, 0, st, g, a->P, a->lam3, a->lam2);
             std::swap(a->lam, a->lam3);
         }
         // the first iterate is phi_0 itself
