@@ -359,19 +359,20 @@ __global__ __launch_bounds__(kAX * kAY) __attribute__((amdgpu_waves_per_eu(3, 8)
 // target[p] = a * self[p] + core[p] + EX[p] + EYlo[p] + EYhi[p] + EZ[p]
 // (+ target[p] if accumulate) over every padded cell p: the block sums of
 // k_adj_zm and its spills.  Planes are zero where nothing was written.  With
-// acc2 given, also acc2[p] += a2 * self[p] (the stage's G_n term).
+// acc2 given, also acc2[p] += a2 * self[p] (the stage's G_n term), or
+// acc2[p] = a2 * self[p] when acc2_init is set (the first term of a step).
 __global__ __launch_bounds__(256) void k_adj_combine(AGeom g, const double* __restrict__ core,
                                                      const double* __restrict__ EX, const double* __restrict__ EYlo,
                                                      const double* __restrict__ EYhi, const double* __restrict__ EZ,
                                                      const double* __restrict__ self, double a,
                                                      double* __restrict__ target, int accumulate,
-                                                     double* __restrict__ acc2, double a2) {
+                                                     double* __restrict__ acc2, double a2, int acc2_init) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < g.n; p += (int64_t)gridDim.x * blockDim.x) {
         const double sp = self ? self[p] : 0.0;
         double acc = self ? a * sp : 0.0;
         acc += core[p] + EX[p] + EYlo[p] + EYhi[p] + EZ[p];
         target[p] = accumulate ? target[p] + acc : acc;
-        if (acc2) acc2[p] += a2 * sp;
+        if (acc2) acc2[p] = acc2_init ? a2 * sp : acc2[p] + a2 * sp;
     }
 }
 
@@ -639,13 +640,15 @@ __global__ __launch_bounds__(kAX * kAY) void k_adj_hcr2_zm(AGeom g, ZMGeom zm, c
 }
 
 // lamN[p] = the phi_n sums (planes 0..3) and lamZ[p] += the phi_0 sums
-// (planes 4..7) over every padded cell p.
+// (planes 4..7; lamZ[p] = them when init is set) over every padded cell p.
 __global__ __launch_bounds__(256) void k_hcr2_combine(AGeom g, const double* __restrict__ planes,
-                                                      double* __restrict__ lamN, double* __restrict__ lamZ) {
+                                                      double* __restrict__ lamN, double* __restrict__ lamZ,
+                                                      int init) {
     const int64_t n = g.n;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         lamN[p] = ((planes[p] + planes[n + p]) + planes[2 * n + p]) + planes[3 * n + p];
-        lamZ[p] += ((planes[4 * n + p] + planes[5 * n + p]) + planes[6 * n + p]) + planes[7 * n + p];
+        const double z = ((planes[4 * n + p] + planes[5 * n + p]) + planes[6 * n + p]) + planes[7 * n + p];
+        lamZ[p] = init ? z : lamZ[p] + z;
     }
 }
 
@@ -949,8 +952,9 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
 
     // Reverse of one stage out = A G_n + B G_s + C dt RHS(G_s) followed by a
     // ghost fill.  lam_in: adjoint of out (its ghosts are folded here).  The
-    // adjoint of G_s goes to lam_s (overwritten); A lam_in is added to acc.
-    // Stage 1 has G_s = G_n: everything goes to acc.
+    // adjoint of G_s goes to lam_s (overwritten); A lam_in is added to acc
+    // (stage 3, the first of a step's reverse, starts acc with it).  Stage 1
+    // has G_s = G_n: everything goes to acc.
     auto stage_vjp = [&](int stage, const double* Gs, double* lam_in, double* lam_s) -> int {
         const double A = stage == 1 ? 1.0 : (stage == 2 ? 0.75 : 1.0 / 3.0);
         const double B = stage == 1 ? 0.0 : (stage == 2 ? 0.25 : 2.0 / 3.0);
@@ -960,10 +964,10 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
                            EX, EYlo, EYhi, EZ, gu, gv, gw, gsl);
         if (stage == 1)
             hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, EX, EYlo, EYhi, EZ, lam_in, A + B, a->acc,
-                               1, (double*)nullptr, 0.0);
+                               1, (double*)nullptr, 0.0, 0);
         else
             hipLaunchKernelGGL(k_adj_combine, lgrid, lblk, 0, st, g, core, EX, EYlo, EYhi, EZ, lam_in, B, lam_s, 0,
-                               a->acc, A);
+                               a->acc, A, stage == 3 ? 1 : 0);
         ADJ_HIP(hipGetLastError());
         return LS_OK;
     };
@@ -971,12 +975,14 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
     // Reverse of a reinitialisation: a->lam holds the adjoint of its output;
     // on return it holds the adjoint of phi_0 (the pre-reinit state).
     auto hcr2_vjp = [&](const double* phi0) -> int {
-        ADJ_HIP(hipMemsetAsync(a->lam2, 0, fb, st));  // adjoint of phi_0 through the r~ and sign terms
+        // lam2: adjoint of phi_0 through the r~ and sign terms, started by the
+        // first (last-iteration) combine
         for (int it = iters - 1; it >= 0; it--) {
             const double* in = it == 0 ? phi0 : a->hit[it];
             hipLaunchKernelGGL(k_fold, fgrid, lblk, 0, st, g, a->lam);
             hipLaunchKernelGGL(k_adj_hcr2_zm, zgrid, cblk, 0, st, g, zm, in, phi0, a->lam, a->P);
-            hipLaunchKernelGGL(k_hcr2_combine, lgrid, lblk, 0, st, g, a->P, a->lam3, a->lam2);
+            hipLaunchKernelGGL(k_hcr2_combine, lgrid, lblk, 0, st, g, a->P, a->lam3, a->lam2,
+                               it == iters - 1 ? 1 : 0);
             std::swap(a->lam, a->lam3);
         }
         // the first iterate is phi_0 itself
@@ -1005,8 +1011,7 @@ int ls_adjoint_run(ls_adjoint_t* a, const double* G0, const double* u, const dou
                 }
                 ADJ_RC(hcr2_vjp(a->pre[q]));
             }
-            ADJ_HIP(hipMemsetAsync(a->acc, 0, fb, st));
-            ADJ_RC(stage_vjp(3, a->g2s[q], a->lam, a->lam2));   // lam2: adjoint of G2
+            ADJ_RC(stage_vjp(3, a->g2s[q], a->lam, a->lam2));   // lam2: adjoint of G2; starts acc
             ADJ_RC(stage_vjp(2, a->g1s[q], a->lam2, a->lam3));  // lam3: adjoint of G1
             ADJ_RC(stage_vjp(1, Gn, a->lam3, nullptr));
             std::swap(a->lam, a->acc);  // adjoint of G_n
