@@ -295,7 +295,12 @@ using DivFma = std::integral_constant<int, lsx::kDivFma>;
 // loads; a sticky block flag in LDS collects the failures) and the host found
 // the spacings in range (z.crdiv); otherwise as divisions.  Both give the
 // same bits, so waves may differ in which they use.
-template <int STAGE, bool EXACT>
+// Launch variants of k_stage_zface (VAR bits): a per-cell S_L field; G_s is
+// G_n (stage 1 of a step: no separate G_n load); every tile full (no bounds
+// tests).  Each drops registers or lane masks the general kernel carries.
+constexpr int kVarSLF = 1, kVarSame = 2, kVarFull = 4;
+
+template <int STAGE, bool EXACT, int VAR>
 __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_stage_zface(
     ZGeom z, const double* __restrict__ Gs, const double* __restrict__ Gn, double* __restrict__ out,
     const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
@@ -305,6 +310,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
     constexpr int FGH = TT::FGH, kFaceThreads = TT::kFaceThreads, kHaloItems = TT::kHaloItems;
     constexpr int NH = TT::NH;
     using FV = typename std::conditional<EXACT, double, double2>::type;  // a stored face
+    constexpr bool SLF = VAR & kVarSLF, SAME = VAR & kVarSame, FULL = VAR & kVarFull;
     __shared__ double sG[FGH][FGW];                      // plane k: tile + 3-cell halo
     __shared__ double sTx[EXACT ? 1 : FY][FX + 4];       // T along x at local x = -2..FX+1
     __shared__ double sTy[EXACT ? 1 : FY + 4][FX];       // T along y at local y = -2..FY+1
@@ -321,11 +327,10 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * FX + lx;
     const int x0 = tx * FX, y0 = ty * FY;
     const int i = x0 + lx, j = y0 + ly;
-    const bool active = i < z.nx && j < z.ny;
+    const bool active = FULL || (i < z.nx && j < z.ny);
     const int kb = ch * z.kz;
     const int ke = min(kb + z.kz, z.nz);
     const int64_t sz = z.sz;
-    const bool same = Gs == Gn;
     const double c2 = 0.25 * dt, c3 = (2.0 / 3.0) * dt;
     bool bad = false;  // exact mode: a value this thread loaded failed crdiv_safe
     if (EXACT) {
@@ -376,14 +381,22 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
             __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(plane), (short)0, -1, 0x00020000);
         return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
     };
+    auto st = [](double* plane, uint32_t off, double val) {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(plane, (short)0, -1, 0x00020000);
+        typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, val), r, (int)off, 0, 0);
+    };
+    // does any lane of this wave own an x/y edge cell (whose ghost images it writes)?
+    const bool wave_edge =
+        __builtin_amdgcn_ballot_w64(active && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3)) != 0;
     auto fetch = [&](int kk, Next& n) {  // plane kk's point data and halo values, and G(kk+3) of the column
         const int64_t pl = (int64_t)(kk + 3) * sz;
         n.lead = ld(Gs + pl + 3 * sz, colb);
         n.u = ld(u + pl, colb);
         n.v = ld(v + pl, colb);
         n.w = ld(w + pl, colb);
-        n.gn = same ? 0.0 : ld(Gn + pl, colb);  // stage 1 passes G_n as the stencil field
-        n.sl = sl ? ld(sl + pl, colb) : z.s_l;
+        if constexpr (!SAME) n.gn = ld(Gn + pl, colb);
+        if constexpr (SLF) n.sl = ld(sl + pl, colb);
 #pragma unroll
         for (int a = 0; a < NH; a++) n.halo[a] = ld(Gs + pl, ohb[a]);
     };
@@ -419,9 +432,10 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         const int64_t plane = (int64_t)(k + 3) * sz;
         zG[6] = nx.lead;
         zt[4] = tsq(zG[4], zG[5], zG[6]);
-        const double cu = nx.u, cv = nx.v, cw = nx.w, cgn = same ? zG[3] : nx.gn, csl = nx.sl;
+        const double cu = nx.u, cv = nx.v, cw = nx.w, cgn = SAME ? zG[3] : nx.gn,
+                     csl = SLF ? nx.sl : z.s_l;
         if (EXACT) {
-            bad |= !lsx::crdiv_safe(zG[6]) || (sl && !lsx::crdiv_safe(csl));
+            bad |= !lsx::crdiv_safe(zG[6]) || (SLF && !lsx::crdiv_safe(csl));
             if (bad) sBad = 1;  // sticky; read after the barrier
         }
         if (k + 1 < ke) fetch(k + 1, nx);
@@ -501,7 +515,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
             const double* g = &sG[ly + 3][lx + 3];
             {  // x face i+1/2, for this cell and its high neighbour
                 bool nL = Lx, nR = !Lx;
-                if (lx + 1 < FX && i + 1 < z.nx) {
+                if (lx + 1 < FX && (FULL || i + 1 < z.nx)) {
                     const bool o = sOri[ly][lx + 1] & 1;
                     nL = nL || o;
                     nR = nR || !o;
@@ -511,7 +525,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
             }
             {  // y face j+1/2
                 bool nL = Ly, nR = !Ly;
-                if (ly + 1 < FY && j + 1 < z.ny) {
+                if (ly + 1 < FY && (FULL || j + 1 < z.ny)) {
                     const bool o = (sOri[ly + 1][lx] >> 1) & 1;
                     nL = nL || o;
                     nR = nR || !o;
@@ -549,13 +563,13 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
             }
         }
         if (ly == kWaveFx && lx < FY) {  // x face on the low edge of row lx, in its tile cell's orientation
-            if (y0 + lx < z.ny) {
+            if (FULL || y0 + lx < z.ny) {
                 const bool o = sOri[lx][0] & 1;
                 put(C1(), C1(), COX(), &sG[lx + 3][2], &sTx[0][0] + (EXACT ? 0 : lx * (FX + 4) + 1), o, !o,
                     &sFx[0][lx][0]);
             }
         } else if (ly == kWaveFy) {  // y face on the low edge of column lx
-            if (x0 + lx < z.nx) {
+            if (FULL || x0 + lx < z.nx) {
                 const bool o = (sOri[0][lx] >> 1) & 1;
                 put(CGW(), CFX(), COY(), &sG[2][lx + 3], &sTy[0][0] + (EXACT ? 0 : FX + lx), o, !o,
                     &sFy[0][0][lx]);
@@ -567,6 +581,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         // the prefetched halo would otherwise also wait for them.
         if (k + 1 < ke) put_plane(zG[4], nx);
         // --- derivatives, RHS, stage combine
+        double o = 0.0;
         if (active && EXACT) {
             // dweno_*_3d: (face(c+1/2) - face(c-1/2)) / h; rhs and stage in the
             // reference's order (rk3.h:31-125)
@@ -585,10 +600,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
                 dgz = az / z.dz;
             }
             const double rr = -(ue * dgx + ve * dgy + we * dgz);
-            const double o = lsx::stage_combine<STAGE>(cgn, zG[3], rr, dt);
-            out[plane + col] = o;
-            if (z.fuse_ghosts && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3))
-                store_images(z, out, i, j, k, o);
+            o = lsx::stage_combine<STAGE>(cgn, zG[3], rr, dt);
         } else if (active) {
             const int ox = Lx ? 0 : 1, oy = Ly ? 0 : 1;
             const double2 xl = as2(sFx[ox][ly][lx]), xh = as2(sFx[ox][ly][lx + 1]);
@@ -601,12 +613,15 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
                                    fma((ve * z.inv6dy) * Ny, Dx * Dz, ((we * z.inv6dz) * Nz) * Dxy));
             const double rr = -(num * rcp_nr(Dxy * Dz));
             const double gc = zG[3];
-            double o;
             if (STAGE == 1) o = fma(dt, rr, cgn);
             else if (STAGE == 2) o = fma(c2, rr, fma(0.25, gc, 0.75 * cgn));
             else o = fma(c3, rr, fma(2.0 / 3.0, gc, (1.0 / 3.0) * cgn));
-            out[plane + col] = o;
-            if (z.fuse_ghosts && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3))
+        }
+        if (active) st(out + plane, colb, o);
+        // ghost images: a scalar test skips them for the waves and planes
+        // that own none
+        if (z.fuse_ghosts && (wave_edge || k < 3 || k >= z.nz - 3)) {
+            if (active && (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3))
                 store_images(z, out, i, j, k, o);
         }
         zlo = zhi;
@@ -772,13 +787,17 @@ __global__ __launch_bounds__(256) void k_hcr2_zmarch(HGeom hg, const double* __r
 // Both iterations use hcr2_point, so the result is bitwise that of two
 // single iterations with a ghost fill between them.  Needs the ghost images
 // of the output (no linear-extrapolation axis).
-constexpr int PQX = HX + 4, PQY = HY + 4;                      // phi_0 plane: tile + 2-cell ring
-constexpr int P1X = HX + 2, P1Y = HY + 2;                      // phi_1 plane: tile + 1-cell ring
-constexpr int kQItems = PQX * PQY, kQPer = (kQItems + HX * HY - 1) / (HX * HY);
-constexpr int kP1Items = P1X * P1Y, kP1Per = (kP1Items + HX * HY - 1) / (HX * HY);
+#ifndef LS_PAIR_ROWS
+#define LS_PAIR_ROWS 8
+#endif
+constexpr int PY = LS_PAIR_ROWS;                               // tile rows of the pair kernel
+constexpr int PQX = HX + 4, PQY = PY + 4;                      // phi_0 plane: tile + 2-cell ring
+constexpr int P1X = HX + 2, P1Y = PY + 2;                      // phi_1 plane: tile + 1-cell ring
+constexpr int kQItems = PQX * PQY, kQPer = (kQItems + HX * PY - 1) / (HX * PY);
+constexpr int kP1Items = P1X * P1Y, kP1Per = (kP1Items + HX * PY - 1) / (HX * PY);
 
-__global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __restrict__ p0,
-                                                    double* __restrict__ pnew) {
+__global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* __restrict__ p0,
+                                                       double* __restrict__ pnew) {
     __shared__ double sQ[5][PQY][PQX];   // phi_0 plane kk in slot (kk + 10) % 5 (kk >= -2)
     __shared__ double sP1[2][P1Y][P1X];  // phi_1 plane kk in slot kk & 1
     const ZGeom& z = hg.z;
@@ -789,7 +808,7 @@ __global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __re
     const int ty = (int)(r % z.tiles_y);
     const int ch = (int)(r / z.tiles_y);
     const int lx = threadIdx.x, ly = threadIdx.y, tid = ly * HX + lx;
-    const int x0 = tx * HX, y0 = ty * HY;
+    const int x0 = tx * HX, y0 = ty * PY;
     const int i = x0 + lx, j = y0 + ly;
     const bool active = i < z.nx && j < z.ny;
     const int kb = ch * z.kz;
@@ -797,13 +816,13 @@ __global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __re
     const int64_t sz = z.sz;
     const bool zgx = z.bc[0] == LS_BC_ZERO_GRAD, zgy = z.bc[1] == LS_BC_ZERO_GRAD, zgz = z.bc[2] == LS_BC_ZERO_GRAD;
 
-    // phi_0 items: local (x, y) in [-2, HX+1] x [-2, HY+1]; positions past the
+    // phi_0 items: local (x, y) in [-2, HX+1] x [-2, PY+1]; positions past the
     // padded array read the nearest padded cell (they feed no interior cell)
     int qx[kQPer], qy[kQPer];
     int32_t qoff[kQPer];
 #pragma unroll
     for (int a = 0; a < kQPer; a++) {
-        const int e = min(tid + a * HX * HY, kQItems - 1);
+        const int e = min(tid + a * HX * PY, kQItems - 1);
         qy[a] = e / PQX;
         qx[a] = e - qy[a] * PQX;
         const int gx = min(x0 - 2 + qx[a], z.nx + 2), gy = min(y0 - 2 + qy[a], z.ny + 2);
@@ -819,14 +838,14 @@ __global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __re
         const int sl = slot(kk);
 #pragma unroll
         for (int a = 0; a < kQPer; a++)
-            if (tid + a * HX * HY < kQItems) sQ[sl][qy[a]][qx[a]] = val[a];
+            if (tid + a * HX * PY < kQItems) sQ[sl][qy[a]][qx[a]] = val[a];
     };
-    // phi_1 items: local (x, y) in [-1, HX] x [-1, HY], evaluated at (ex, ey)
+    // phi_1 items: local (x, y) in [-1, HX] x [-1, PY], evaluated at (ex, ey)
     // (sQ indices; the clamped cell on a zero-gradient axis)
     int px[kP1Per], py[kP1Per], ex[kP1Per], ey[kP1Per];
 #pragma unroll
     for (int a = 0; a < kP1Per; a++) {
-        const int e = min(tid + a * HX * HY, kP1Items - 1);
+        const int e = min(tid + a * HX * PY, kP1Items - 1);
         py[a] = e / P1X;
         px[a] = e - py[a] * P1X;
         int gx = x0 - 1 + px[a], gy = y0 - 1 + py[a];
@@ -852,7 +871,7 @@ __global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __re
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < kP1Per; a++)
-        if (tid + a * HX * HY < kP1Items) sP1[kb & 1][py[a]][px[a]] = phi1(zev(kb), ex[a], ey[a]);
+        if (tid + a * HX * PY < kP1Items) sP1[kb & 1][py[a]][px[a]] = phi1(zev(kb), ex[a], ey[a]);
     double fm = phi1(zev(kb - 1), lx + 2, ly + 2);  // phi_1 of this column one plane below
 
     for (int k = kb; k < ke; k++) {
@@ -860,7 +879,7 @@ __global__ __launch_bounds__(256) void k_hcr2_pair0(HGeom hg, const double* __re
         __syncthreads();  // phi_0 plane k+2 in place; phi_1 plane k-1 no longer read
 #pragma unroll
         for (int a = 0; a < kP1Per; a++)
-            if (tid + a * HX * HY < kP1Items) sP1[(k + 1) & 1][py[a]][px[a]] = phi1(zev(k + 1), ex[a], ey[a]);
+            if (tid + a * HX * PY < kP1Items) sP1[(k + 1) & 1][py[a]][px[a]] = phi1(zev(k + 1), ex[a], ey[a]);
         __syncthreads();  // phi_1 plane k+1 in place
         // phi_0 plane k+3 into the slot of plane k-2, last read in the previous
         // plane, before this plane's stores: vmcnt counts stores too, so a wait
@@ -942,6 +961,25 @@ static ZGeom make_z(const Geom& g, int kz, int tile_cols = ZX, int tile_rows = Z
     return z;
 }
 
+// Launches stage `stage` of k_stage_zface with variant bits V (kVarSame is
+// added for stage 1 when G_s is G_n; stages 2 and 3 always load G_n, which is
+// also correct when the caller passes the same array).
+template <bool E, int V>
+static void zface_stage(int stage, bool same, dim3 grid, dim3 block, hipStream_t st, const ZGeom& z,
+                        const double* Gs, const double* Gn, double* out, const double* u, const double* v,
+                        const double* w, const double* sl, double dt) {
+    if (stage == 1) {
+        if (same)
+            hipLaunchKernelGGL((k_stage_zface<1, E, V | kVarSame>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        else
+            hipLaunchKernelGGL((k_stage_zface<1, E, V>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+    } else if (stage == 2) {
+        hipLaunchKernelGGL((k_stage_zface<2, E, V>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+    } else {
+        hipLaunchKernelGGL((k_stage_zface<3, E, V>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+    }
+}
+
 // Returns 1 when the launched kernel also filled the ghost cells of out.
 int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double* Gn, double* out,
                         const double* u, const double* v, const double* w, const double* sl, double dt,
@@ -953,19 +991,20 @@ int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double
     const ZGeom z = make_z(g, kz, FX, FY);
     if (z.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
     const dim3 grid((unsigned)z.nblocks), block(FX, FY, 1);
-#define LS_ZFACE_LAUNCH(E)                                                                                      \
-    if (stage == 1)                                                                                             \
-        hipLaunchKernelGGL((k_stage_zface<1, E>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);        \
-    else if (stage == 2)                                                                                        \
-        hipLaunchKernelGGL((k_stage_zface<2, E>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);        \
-    else                                                                                                        \
-        hipLaunchKernelGGL((k_stage_zface<3, E>), grid, block, 0, st, z, Gs, Gn, out, u, v, w, sl, dt);
+    const bool same = stage == 1 && Gs == Gn;
     if (exact) {
-        LS_ZFACE_LAUNCH(true)
+        if (sl) zface_stage<true, kVarSLF>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        else zface_stage<true, 0>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
     } else {
-        LS_ZFACE_LAUNCH(false)
+        const bool full = g.nx % FX == 0 && g.ny % FY == 0;
+        if (full) {
+            if (sl) zface_stage<false, kVarSLF | kVarFull>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+            else zface_stage<false, kVarFull>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        } else {
+            if (sl) zface_stage<false, kVarSLF>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+            else zface_stage<false, 0>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        }
     }
-#undef LS_ZFACE_LAUNCH
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "z-march stage launch failed");
     return z.fuse_ghosts ? 1 : 0;
 }
@@ -1078,7 +1117,7 @@ int launch_hcr2_pair_fast(const Geom& g, const double* p0, double* pnew, hipStre
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
         return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
     HGeom hg;
-    hg.z = make_z(g, zmarch_chunk(g, HX, HY));
+    hg.z = make_z(g, zmarch_chunk(g, HX, PY), HX, PY);
     if (!hg.z.fuse_ghosts) return 0;
     hg.dx2 = g.dx * g.dx;
     hg.invdx = 1.0 / g.dx;
@@ -1087,7 +1126,7 @@ int launch_hcr2_pair_fast(const Geom& g, const double* p0, double* pnew, hipStre
     hg.dtau = g.dtau;
     hg.beta = g.beta;
     if (hg.z.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
-    hipLaunchKernelGGL(k_hcr2_pair0, dim3((unsigned)hg.z.nblocks), dim3(HX, HY, 1), 0, st, hg, p0, pnew);
+    hipLaunchKernelGGL(k_hcr2_pair0, dim3((unsigned)hg.z.nblocks), dim3(HX, PY, 1), 0, st, hg, p0, pnew);
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast HCR-2 pair launch failed");
     return 1;
 }
