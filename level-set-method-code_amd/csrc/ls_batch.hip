@@ -99,7 +99,9 @@ void k_batch_ymarch(const BMember* __restrict__ mem, const BTile* __restrict__ t
         m.rdy = b.rdy;
         m.crdiv = b.crdiv;
         const int y0 = tl.ty * ch;
-        lsf::ymarch_strip<STAGE, EXACT>(m, tl.tx * lsf::kYW, y0, min(y0 + ch, b.ny), Gs, Gn, out, u, v, nullptr, L);
+        // the engine passes G_s == G_n exactly for stage 1
+        lsf::ymarch_strip<STAGE, EXACT, STAGE == 1 ? 1 : 0>(m, tl.tx * lsf::kYW, y0, min(y0 + ch, b.ny), Gs, Gn,
+                                                             out, u, v, nullptr, L);
     }
 }
 

@@ -167,8 +167,10 @@ __device__ __forceinline__ void ycell(const YGrid& m, int i, int j, const double
 // the exact mode the divisions run as the FMA sequences of ls_crdiv.hpp while
 // every G and S_L value the wavefront has loaded passed lsx::crdiv_safe (a
 // sticky per-lane flag and a wave vote) and m.crdiv is set; otherwise as
-// divisions.  Both give the same bits.
-template <int STAGE, bool EXACT = false>
+// divisions.  Both give the same bits.  SM says whether G_s is G_n: -1 test
+// the pointers, 1 known equal (stage 1 of a step: no separate G_n load), 0
+// known different.
+template <int STAGE, bool EXACT = false, int SM = -1>
 __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int y1,
                                              const double* __restrict__ Gs, const double* __restrict__ Gn,
                                              double* __restrict__ out, const double* __restrict__ u,
@@ -184,7 +186,7 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
     const bool hl = l < 6;
     const int hp = l < 3 ? l : xe - x0 + l;  // LDS position of the halo value
     const int64_t hcol = m.off + (l < 3 ? x0 + l : xe + l);
-    const bool same = Gs == Gn;  // stage 1: the combine's G_n is the stencil's centre
+    const bool same = SM < 0 ? Gs == Gn : SM == 1;  // the combine's G_n is the stencil's centre
     auto prow = [&](int j) { return (int64_t)(j + 3) * sy; };
 
     double r[7];  // G(i, j-3 .. j+3)
@@ -230,6 +232,19 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         }
         return q;
     };
+    auto st = [&](double* row, uint32_t off, double val) {
+        if constexpr (EXACT) {
+            *reinterpret_cast<double*>(reinterpret_cast<char*>(row) + off) = val;
+        } else {
+            const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(row, (short)0, -1, 0x00020000);
+            typedef unsigned int u32x2 __attribute__((__vector_size__(8)));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, val), r, (int)off, 0, 0);
+        }
+    };
+    // does any lane of this strip own a cell next to an x edge (whose ghost
+    // images it writes)?  Rows away from the y edges then skip the images
+    // with a scalar test.
+    const bool strip_xedge = __ballot(act && (i < 3 || i >= m.nx - 3)) != 0;
     YRow cur = fetch(y0, false);  // G(i, y0 + 3) is r[6]
     YRow ahead = cur;             // row j + 1 when prefetching two rows ahead
     if (LS_YMARCH_PF == 2) ahead = fetch(y0 + 1, true);
@@ -337,10 +352,10 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         }
         __syncthreads();  // faces
         // --- derivatives over one denominator, RHS, stage combine
+        double o2 = 0.0;
         if (act) {
             const int o = Lx ? 0 : 1;
             const double2 fl = L.f[o][l], fh = L.f[o][l + 1];
-            double o2;
             if (EXACT) {  // (F(i+1/2) - F(i-1/2)) / h, rhs and the stage in the reference's order
                 const double ax = fh.x - fl.x, ay = yhi.n - ylo.n;
                 const double dgx = fd ? lsx::qdiv_u(ax, m.dx, m.rdx) : ax / m.dx;
@@ -356,9 +371,11 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
                 const double rr = yrhs(m, xl, xh, ylo, yhi, ue, ve);
                 o2 = ystage<STAGE>(same ? r[3] : cur.gn, r[3], rr, m.dt);
             }
-            out[col + prow(j)] = o2;
-            // ghost cells of out: written by the interior cell they copy
-            ystore_images(m, out, i, j, o2);
+            st(out + base + prow(j), cb, o2);
+        }
+        // ghost cells of out: written by the interior cell they copy
+        if (m.fuse_ghosts && (strip_xedge || j < 3 || j >= m.ny - 3)) {
+            if (act) ystore_images(m, out, i, j, o2);
         }
         ylo = yhi;
         ylo_L = (int)Ly;
