@@ -992,11 +992,16 @@ int launch_stage_zmarch(const Geom& g, int stage, const double* Gs, const double
     if (z.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
     const dim3 grid((unsigned)z.nblocks), block(FX, FY, 1);
     const bool same = stage == 1 && Gs == Gn;
+    const bool full = g.nx % FX == 0 && g.ny % FY == 0;
     if (exact) {
-        if (sl) zface_stage<true, kVarSLF>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
-        else zface_stage<true, 0>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        if (full) {
+            if (sl) zface_stage<true, kVarSLF | kVarFull>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+            else zface_stage<true, kVarFull>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        } else {
+            if (sl) zface_stage<true, kVarSLF>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+            else zface_stage<true, 0>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
+        }
     } else {
-        const bool full = g.nx % FX == 0 && g.ny % FY == 0;
         if (full) {
             if (sl) zface_stage<false, kVarSLF | kVarFull>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
             else zface_stage<false, kVarFull>(stage, same, grid, block, st, z, Gs, Gn, out, u, v, w, sl, dt);
