@@ -26,6 +26,8 @@ Prints ONE JSON line (rank 0).  Options:
                            WRITE_SIZE) into profiles/stage_kernel_traffic_512_<mode>.json
   --summarize-sq D...      SQ counters of the stage kernels from rocprofv3 --pmc
                            CSV directories, per wave-plane, as a markdown table
+                           (--sq-kernel NAME --sq-cells N for another kernel:
+                           per 64 cells of an N-cell dispatch)
   --config batch           the batched-ensemble workload instead: 96 2D
                            flames of seeded sizes in {128,192,256,384,512}^2,
                            S_L ~ U[0.05,0.2], 300 RK3 steps each, in one
@@ -71,6 +73,12 @@ def traffic_file(mode):
 # variant 0 = scalar S_L (its 64x8 tiles have no full-tile variant)
 FAST_STAGE_KERNEL = "k_stage_zface<2, false, 4>"
 EXACT_STAGE_KERNEL = "k_stage_zface<2, true, 0>"
+
+
+def progress(msg):
+    """A progress line on stderr (long phases under a profiler stay visible)."""
+    sys.stderr.write("[bench] %s\n" % msg)
+    sys.stderr.flush()
 
 
 def load_ls():
@@ -220,6 +228,7 @@ def ensemble(ls, count=96, seed=4, mode="fast"):
 def bench_batch(ls, steps, mode, emit=True):
     import torch
     members, params, fields = ensemble(ls, mode=mode)
+    progress("batch: %d members built" % len(members))
     b = ls.Batch(members, mode=ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT)
     G, U, V = b.empty(), b.empty(), b.empty()
     for q, (g, u, v) in enumerate(fields):
@@ -237,6 +246,7 @@ def bench_batch(ls, steps, mode, emit=True):
     b.advance(G, U, V, steps, graph=True)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    progress("batch: timed run done")
     cells = sum(m["nx"] * m["ny"] for m in members)
     # serial per-instance loop over the single-grid solver
     solvers = [ls.Solver(p) for p in params]
@@ -440,6 +450,7 @@ def bench_sample(ls, steps, n=800):
     reference's own sample code (OpenMP, 16 threads) timed beside it."""
     import torch
     s = ls.Sample(n, n)
+    progress("sample: %dx%d set up" % (n, n))
     G0 = s.bunsen()
     G = G0.clone()
     u, v = s.empty(), s.empty()
@@ -449,6 +460,7 @@ def bench_sample(ls, steps, n=800):
     t = s.advance(G, u, v, G0, steps, freq=10, t0=t)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    progress("sample: timed loop done")
     # RK3 kernels alone (3 stage launches), HIP events on the launch stream
     st = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -567,6 +579,34 @@ def profile_pass(ls, n, mode):
     print("profile pass done", flush=True)
 
 
+def profile_pass_2d(ls, config, mode):
+    """Kernels for --pmc runs of the 2D workloads, a few launches each (every
+    dispatch is serialised and slow under counter collection): one RK3 step of
+    the sample/ scheme on its 800^2 Bunsen case, or two RK3 steps of the
+    96-member ensemble on its 8-row y-march chunks (schedule shape 3)."""
+    import torch
+    if config == "sample":
+        s = ls.Sample(800, 800)
+        G0 = s.bunsen()
+        G = G0.clone()
+        u, v = s.empty(), s.empty()
+        s.forced_velocity(0.0, 10, u, v)
+        s.rk3_step(G, u, v, 2.5e-6, 0.4, G0)
+    else:
+        members, params, fields = ensemble(ls, mode=mode)
+        b = ls.Batch(members, mode=ls.MODE_FAST if mode == "fast" else ls.MODE_EXACT)
+        G, U, V = b.empty(), b.empty(), b.empty()
+        for q, (g, u, v) in enumerate(fields):
+            b.view(G, q).copy_(g)
+            b.view(U, q).copy_(u)
+            b.view(V, q).copy_(v)
+        b.fill_ghosts(G)
+        b.set_tile(3)
+        b.advance(G, U, V, 2, graph=False)
+    torch.cuda.synchronize()
+    print("profile pass done", flush=True)
+
+
 def _pmc_rows(d):
     rows = []
     for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
@@ -614,27 +654,29 @@ def summarize_pmc(fetch_dir, write_dir, n, mode, out_path=None):
     print(json.dumps(d))
 
 
-def summarize_sq(dirs, n):
-    """SQ counters (rocprofv3 --pmc CSV directories) of the 3D stage kernels as
-    a markdown table, averaged over dispatches and divided by the wave-planes
-    of one launch (n^3 / 64 cells: one 64-wide wavefront row per plane)."""
+def summarize_sq(dirs, n, kernel="k_stage_zface", cells=None):
+    """SQ counters (rocprofv3 --pmc CSV directories) of the kernels whose name
+    contains `kernel`, as a markdown table: each counter averaged over the
+    kernel's dispatches and divided by the 64-cell units of one dispatch
+    (`cells` / 64; by default the n^3 cells of the 3D stage launch, i.e. one
+    64-wide wavefront row per plane)."""
     vals = {}
     for d in dirs:
         for r in _pmc_rows(d):
-            k = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
-            if "k_stage_zface" not in k:
+            k = r["Kernel_Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").strip()
+            if kernel not in k:
                 continue
             vals.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-    wp = n ** 3 / 64.0
+    wp = (cells if cells else n ** 3) / 64.0
     names = sorted({c for v in vals.values() for c in v})
-    print("| counter | " + " | ".join(sorted(vals)) + " |")
+    print("| counter, per 64 cells | " + " | ".join(sorted(vals)) + " |")
     print("|---|" + "---|" * len(vals))
     for c in names:
         row = []
         for k in sorted(vals):
             x = vals[k].get(c)
             row.append("%.2f" % (sum(x) / len(x) / wp) if x else "")
-        print("| %s per wave-plane | %s |" % (c, " | ".join(row)))
+        print("| %s | %s |" % (c, " | ".join(row)))
 
 
 def timed_steps(ls, n, mode, steps, warmup):
@@ -688,6 +730,10 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--traffic-out", default=None)
     ap.add_argument("--summarize-sq", nargs="+", metavar="PMC_DIR")
+    ap.add_argument("--sq-kernel", default="k_stage_zface",
+                    help="with --summarize-sq: kernel name substring to tabulate")
+    ap.add_argument("--sq-cells", type=int, default=None,
+                    help="with --summarize-sq: cells per dispatch (default n^3)")
     ap.add_argument("--config", choices=["plane_modes", "batch", "adjoint", "sample", "zalesak"],
                     default="plane_modes")
     ap.add_argument("--sets", type=int, default=32)
@@ -701,7 +747,7 @@ def main():
         summarize_pmc(a.summarize_pmc[0], a.summarize_pmc[1], a.n, a.mode, a.traffic_out)
         return
     if a.summarize_sq:
-        summarize_sq(a.summarize_sq, a.n)
+        summarize_sq(a.summarize_sq, a.n, a.sq_kernel, a.sq_cells)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus > 1 or world > 1:
@@ -713,7 +759,10 @@ def main():
     ls = load_ls()
     torch.cuda.set_device(0)
     if a.profile_pass:
-        profile_pass(ls, a.n, a.mode)
+        if a.config in ("sample", "batch"):
+            profile_pass_2d(ls, a.config, a.mode)
+        else:
+            profile_pass(ls, a.n, a.mode)
         return
     if a.config == "batch":
         bench_batch(ls, 300 if a.steps == 20 else a.steps, a.mode)
