@@ -161,20 +161,22 @@ __device__ __forceinline__ double stage_combine(double gn, double gs, double r, 
     return (1.0 / 3.0) * gn + (2.0 / 3.0) * gs + (2.0 / 3.0) * dt * r;
 }
 
-// One HCR-2 pseudo-time step at a cell.  Crossing flags and r~ are recomputed
-// from phi_0 (identical arithmetic to computeInterfaceCrossings, so identical
-// values to the stored arrays of the reference).
+// One HCR-2 pseudo-time step at a cell from phi_n (P) and phi_0 (Q) at the
+// cell and its neighbours in flag order x-, x+, y-, y+, z-, z+ (entries 5 and
+// 6 unused in 2D), in the reference's operation order (reinitialization.h,
+// 3D :136-193).  Crossing flags and r~ are recomputed from phi_0 with the
+// arithmetic of computeInterfaceCrossings, so they equal the reference's
+// stored arrays.
 template <int DIM>
-__device__ __forceinline__ double hcr2_cell(const Geom& g, const double* __restrict__ pn,
-                                            const double* __restrict__ p0) {
-    const int64_t off[6] = {-1, 1, -g.sy, g.sy, -g.sz, g.sz};
+__device__ __forceinline__ double hcr2_core(const double (&P)[7], const double (&Q)[7], double dx, double dy,
+                                            double dz, double dtau, double beta) {
     constexpr int NB = DIM == 3 ? 6 : 4;
-    const double pc = p0[0];
+    const double pc = Q[0];
     double sum0 = 0.0;
     int cnt = 0, flag = 0;
 #pragma unroll
     for (int q = 0; q < NB; q++) {
-        double pq = p0[off[q]];
+        const double pq = Q[q + 1];
         if (pc * pq < 0.0) {
             sum0 += pq;
             cnt++;
@@ -184,14 +186,14 @@ __device__ __forceinline__ double hcr2_cell(const Geom& g, const double* __restr
     double rt = 0.0;
     if (cnt > 0 && fabs(sum0) > 1e-15) rt = pc / sum0;
 
-    const double phi = pn[0];
-    const double S = pc / sqrt(pc * pc + g.dx * g.dx);
-    double xm = (pn[0] - pn[-1]) / g.dx, xp = (pn[1] - pn[0]) / g.dx;
-    double ym = (pn[0] - pn[-g.sy]) / g.dy, yp = (pn[g.sy] - pn[0]) / g.dy;
+    const double phi = P[0];
+    const double S = pc / sqrt(pc * pc + dx * dx);
+    double xm = (P[0] - P[1]) / dx, xp = (P[2] - P[0]) / dx;
+    double ym = (P[0] - P[3]) / dy, yp = (P[4] - P[0]) / dy;
     double zm = 0.0, zp = 0.0;
     if (DIM == 3) {
-        zm = (pn[0] - pn[-g.sz]) / g.dz;
-        zp = (pn[g.sz] - pn[0]) / g.dz;
+        zm = (P[0] - P[5]) / dz;
+        zp = (P[6] - P[0]) / dz;
     }
     double ax, bx, ay, by, az, bz;
     if (S > 0.0) {
@@ -210,19 +212,35 @@ __device__ __forceinline__ double hcr2_cell(const Geom& g, const double* __restr
 
     double forcing = 0.0;
     if (flag != 0) {
-        bool stable = (pn[0] > 0.0) == (pc > 0.0);
+        bool stable = (P[0] > 0.0) == (pc > 0.0);
 #pragma unroll
         for (int q = 0; q < NB; q++)
-            if ((flag >> q) & 1) stable = stable && ((pn[off[q]] > 0.0) == (p0[off[q]] > 0.0));
+            if ((flag >> q) & 1) stable = stable && ((P[q + 1] > 0.0) == (Q[q + 1] > 0.0));
         if (stable) {
             double sum = 0.0;
 #pragma unroll
             for (int q = 0; q < NB; q++)
-                if ((flag >> q) & 1) sum += pn[off[q]];
-            forcing = g.beta * (rt * sum - phi) / g.dx;
+                if ((flag >> q) & 1) sum += P[q + 1];
+            forcing = beta * (rt * sum - phi) / dx;
         }
     }
-    return phi + g.dtau * (term + forcing);
+    return phi + dtau * (term + forcing);
+}
+
+// hcr2_core at the cell pn / p0 point at (padded layout of g).
+template <int DIM>
+__device__ __forceinline__ double hcr2_cell(const Geom& g, const double* __restrict__ pn,
+                                            const double* __restrict__ p0) {
+    const int64_t off[6] = {-1, 1, -g.sy, g.sy, -g.sz, g.sz};
+    double P[7], Q[7];
+    P[0] = pn[0];
+    Q[0] = p0[0];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        P[q + 1] = (DIM == 3 || q < 4) ? pn[off[q]] : 0.0;
+        Q[q + 1] = (DIM == 3 || q < 4) ? p0[off[q]] : 0.0;
+    }
+    return hcr2_core<DIM>(P, Q, g.dx, g.dy, g.dz, g.dtau, g.beta);
 }
 
 }  // namespace lsx

@@ -784,9 +784,11 @@ __global__ __launch_bounds__(256) void k_hcr2_zmarch(HGeom hg, const double* __r
 //  * a periodic axis computes it in place, because phi_0 is periodic over the
 //    padded array, so the stencil there equals the stencil of the image;
 //  * a zero-gradient axis takes it from the clamped cell.
-// Both iterations use hcr2_point, so the result is bitwise that of two
-// single iterations with a ghost fill between them.  Needs the ghost images
-// of the output (no linear-extrapolation axis).
+// Both iterations use the single iteration's point update (hcr2_point in the
+// fast mode, the reference's arithmetic lsx::hcr2_core in the exact mode), so
+// the result is bitwise that of two single iterations with a ghost fill
+// between them.  Needs the ghost images of the output (no
+// linear-extrapolation axis).
 // Tiles are 64x8 (8 waves): the first iteration then covers 66x10 cells per
 // 512 outputs, 1.29 per cell, against 66x6 per 256 (1.55) on 64x4 tiles
 // (1.53 against 1.66 ms per pair at 512^3; 64x16 tiles were slower).
@@ -799,6 +801,7 @@ constexpr int P1X = HX + 2, P1Y = PY + 2;                      // phi_1 plane: t
 constexpr int kQItems = PQX * PQY, kQPer = (kQItems + HX * PY - 1) / (HX * PY);
 constexpr int kP1Items = P1X * P1Y, kP1Per = (kP1Items + HX * PY - 1) / (HX * PY);
 
+template <bool EXACT>
 __global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* __restrict__ p0,
                                                        double* __restrict__ pnew) {
     __shared__ double sQ[5][PQY][PQX];   // phi_0 plane kk in slot (kk + 10) % 5 (kk >= -2)
@@ -818,6 +821,10 @@ __global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* 
     const int ke = min(kb + z.kz, z.nz);
     const int64_t sz = z.sz;
     const bool zgx = z.bc[0] == LS_BC_ZERO_GRAD, zgy = z.bc[1] == LS_BC_ZERO_GRAD, zgz = z.bc[2] == LS_BC_ZERO_GRAD;
+    auto point = [&](const double (&P)[7], const double (&Q)[7]) {
+        if constexpr (EXACT) return lsx::hcr2_core<3>(P, Q, z.dx, z.dy, z.dz, hg.dtau, hg.beta);
+        else return hcr2_point(hg, P, Q);
+    };
 
     // phi_0 items: local (x, y) in [-2, HX+1] x [-2, PY+1]; positions past the
     // padded array read the nearest padded cell (they feed no interior cell)
@@ -863,7 +870,7 @@ __global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* 
         const int sc = slot(kk), sm = slot(kk - 1), sp = slot(kk + 1);
         const double Q[7] = {sQ[sc][Y][X],     sQ[sc][Y][X - 1], sQ[sc][Y][X + 1], sQ[sc][Y - 1][X],
                              sQ[sc][Y + 1][X], sQ[sm][Y][X],     sQ[sp][Y][X]};
-        return hcr2_point(hg, Q, Q);
+        return point(Q, Q);
     };
 
     double val[kQPer];
@@ -901,7 +908,7 @@ __global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* 
             const int sc = slot(k), QX = lx + 2, QY = ly + 2;
             const double Q[7] = {sQ[sc][QY][QX],     sQ[sc][QY][QX - 1], sQ[sc][QY][QX + 1], sQ[sc][QY - 1][QX],
                                  sQ[sc][QY + 1][QX], sQ[slot(k - 1)][QY][QX], sQ[slot(k + 1)][QY][QX]};
-            const double o = hcr2_point(hg, P, Q);
+            const double o = point(P, Q);
             const int64_t c = (int64_t)(k + 3) * sz + (int64_t)(j + 3) * z.sy + (i + 3);
             pnew[c] = o;
             if (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3)
@@ -1117,10 +1124,11 @@ template int launch_stage_fast<2>(const Geom&, int, const double*, const double*
 template int launch_stage_fast<3>(const Geom&, int, const double*, const double*, double*, const double*,
                                   const double*, const double*, const double*, double, hipStream_t);
 
-// The first two HCR-2 iterations from phi_0 in one launch (3D).  Returns 1
-// when it ran (the output's ghost cells included), 0 when this grid needs the
-// single-iteration path (a linear-extrapolation axis).
-int launch_hcr2_pair_fast(const Geom& g, const double* p0, double* pnew, hipStream_t st) {
+// The first two HCR-2 iterations from phi_0 in one launch (3D), in the fast
+// or the exact arithmetic.  Returns 1 when it ran (the output's ghost cells
+// included), 0 when this grid needs the single-iteration path (a
+// linear-extrapolation axis).
+int launch_hcr2_pair(const Geom& g, const double* p0, double* pnew, hipStream_t st, bool exact) {
     if (g.dim != 3) return 0;
     if (g.sz * (int64_t)g.nzt >= 0x7fffffffLL)
         return lsi::set_error(LS_ERR_INVALID, "padded grid exceeds 2^31 cells");
@@ -1134,8 +1142,11 @@ int launch_hcr2_pair_fast(const Geom& g, const double* p0, double* pnew, hipStre
     hg.dtau = g.dtau;
     hg.beta = g.beta;
     if (hg.z.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
-    hipLaunchKernelGGL(k_hcr2_pair0, dim3((unsigned)hg.z.nblocks), dim3(HX, PY, 1), 0, st, hg, p0, pnew);
-    if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast HCR-2 pair launch failed");
+    if (exact)
+        hipLaunchKernelGGL(k_hcr2_pair0<true>, dim3((unsigned)hg.z.nblocks), dim3(HX, PY, 1), 0, st, hg, p0, pnew);
+    else
+        hipLaunchKernelGGL(k_hcr2_pair0<false>, dim3((unsigned)hg.z.nblocks), dim3(HX, PY, 1), 0, st, hg, p0, pnew);
+    if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "HCR-2 pair launch failed");
     return 1;
 }
 

@@ -22,5 +22,7 @@ int launch_stage_ymarch(const lsx::Geom& g, int stage, const double* Gs, const d
                         const double* u, const double* v, const double* sl, double dt, hipStream_t st, bool exact);
 int launch_hcr2_fast(const lsx::Geom& g, const double* pn, const double* p0, double* pnew,
                      hipStream_t st);
-int launch_hcr2_pair_fast(const lsx::Geom& g, const double* p0, double* pnew, hipStream_t st);
+// The first two HCR-2 iterations in one z-march (3D), fast or exact arithmetic;
+// returns 1 when it ran, 0 when the grid needs single iterations.
+int launch_hcr2_pair(const lsx::Geom& g, const double* p0, double* pnew, hipStream_t st, bool exact);
 }  // namespace lsf

@@ -471,9 +471,9 @@ static int hcr2_rotate(const ls_solver* s, double** cur, double** a, double** b,
     double* src = p0;
     double* dst = *a;
     int it = 0;
-    if (s->p.mode == LS_MODE_FAST && iters >= 2) {
+    if (iters >= 2) {
         // the first two iterations in one pass where the grid allows it
-        int rc = lsf::launch_hcr2_pair_fast(s->g, p0, *b, st);
+        int rc = lsf::launch_hcr2_pair(s->g, p0, *b, st, s->p.mode != LS_MODE_FAST);
         if (rc < 0) return rc;
         if (rc == 1) {
             src = *b;
