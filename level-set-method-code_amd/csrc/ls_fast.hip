@@ -421,6 +421,15 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
 #pragma unroll
     for (int q = 0; q < 4; q++) zt[q] = tsq(zG[q], zG[q + 1], zG[q + 2]);
     zt[4] = 0.0;
+    // The fast mode's constants as vector-register copies: the VALU reads them
+    // there as well as from scalar registers, and the scalar registers they
+    // free hold loop-invariant lane masks that would otherwise spill.  (The
+    // empty asm only pins each value to a vector register.)
+    double vi2x = z.inv2dx, vi2y = z.inv2dy, vi2z = z.inv2dz, vi6x = z.inv6dx, vi6y = z.inv6dy, vi6z = z.inv6dz;
+    double vsl = z.s_l, veps = z.eps, vc = STAGE == 1 ? dt : STAGE == 2 ? c2 : c3;
+    if constexpr (!EXACT)
+        asm volatile("" : "+v"(vi2x), "+v"(vi2y), "+v"(vi2z), "+v"(vi6x), "+v"(vi6y), "+v"(vi6z), "+v"(vsl),
+                     "+v"(veps), "+v"(vc));
     Next nx;
     fetch(kb, nx);
     put_plane(zG[3], nx);
@@ -433,7 +442,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         zG[6] = nx.lead;
         zt[4] = tsq(zG[4], zG[5], zG[6]);
         const double cu = nx.u, cv = nx.v, cw = nx.w, cgn = SAME ? zG[3] : nx.gn,
-                     csl = SLF ? nx.sl : z.s_l;
+                     csl = SLF ? nx.sl : (EXACT ? z.s_l : vsl);
         if (EXACT) {
             bad |= !lsx::crdiv_safe(zG[6]) || (SLF && !lsx::crdiv_safe(csl));
             if (bad) sBad = 1;  // sticky; read after the barrier
@@ -474,12 +483,12 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
             };
             if (fd) ueff(DivFma());
             else ueff(DivIEEE());
-        } else if (csl > z.eps) {
-            const double gx = (sG[ly + 3][lx + 4] - sG[ly + 3][lx + 2]) * z.inv2dx;
-            const double gy = (sG[ly + 4][lx + 3] - sG[ly + 2][lx + 3]) * z.inv2dy;
-            const double gz = (zG[4] - zG[2]) * z.inv2dz;
+        } else if (csl > veps) {
+            const double gx = (sG[ly + 3][lx + 4] - sG[ly + 3][lx + 2]) * vi2x;
+            const double gy = (sG[ly + 4][lx + 3] - sG[ly + 2][lx + 3]) * vi2y;
+            const double gz = (zG[4] - zG[2]) * vi2z;
             const double s2 = fma(gz, gz, fma(gx, gx, gy * gy));
-            const double kk = csl * rsqrt_nr(s2 + z.eps);
+            const double kk = csl * rsqrt_nr(s2 + veps);
             ue = fma(-kk, gx, ue);
             ve = fma(-kk, gy, ve);
             we = fma(-kk, gz, we);
@@ -609,13 +618,13 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
             const double Ny = fma(yh.x, yl.y, -(yl.x * yh.y)), Dy = yh.y * yl.y;
             const double Nz = fma(zhi.n, zlo.d, -(zlo.n * zhi.d)), Dz = zhi.d * zlo.d;
             const double Dxy = Dx * Dy;
-            const double num = fma((ue * z.inv6dx) * Nx, Dy * Dz,
-                                   fma((ve * z.inv6dy) * Ny, Dx * Dz, ((we * z.inv6dz) * Nz) * Dxy));
+            const double num = fma((ue * vi6x) * Nx, Dy * Dz,
+                                   fma((ve * vi6y) * Ny, Dx * Dz, ((we * vi6z) * Nz) * Dxy));
             const double rr = -(num * rcp_nr(Dxy * Dz));
             const double gc = zG[3];
-            if (STAGE == 1) o = fma(dt, rr, cgn);
-            else if (STAGE == 2) o = fma(c2, rr, fma(0.25, gc, 0.75 * cgn));
-            else o = fma(c3, rr, fma(2.0 / 3.0, gc, (1.0 / 3.0) * cgn));
+            if (STAGE == 1) o = fma(vc, rr, cgn);
+            else if (STAGE == 2) o = fma(vc, rr, fma(0.25, gc, 0.75 * cgn));
+            else o = fma(vc, rr, fma(2.0 / 3.0, gc, (1.0 / 3.0) * cgn));
         }
         if (active) st(out + plane, colb, o);
         // ghost images: a scalar test skips them for the waves and planes
