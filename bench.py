@@ -69,10 +69,9 @@ def traffic_file(mode):
     """Measured HBM bytes of the stage-2 kernel at 512^3, one file per arithmetic mode."""
     return os.path.join(ROOT, "profiles", "stage_kernel_traffic_512_%s.json" % mode)
 # the 3D stage-2 kernels (shared-face z-march) the 512^3 workload launches,
-# fast and exact arithmetic: variant 4 = full tiles, scalar S_L; exact mode
-# variant 0 = scalar S_L (its 64x8 tiles have no full-tile variant)
+# fast and exact arithmetic: variant 4 = full tiles, scalar S_L
 FAST_STAGE_KERNEL = "k_stage_zface<2, false, 4>"
-EXACT_STAGE_KERNEL = "k_stage_zface<2, true, 0>"
+EXACT_STAGE_KERNEL = "k_stage_zface<2, true, 4>"
 
 
 def progress(msg):
