@@ -557,7 +557,7 @@ def triad_bandwidth(reps=10):
 def profile_pass(ls, n, mode):
     """Kernels for the --pmc runs: the calibration triad over 1 GiB arrays (the
     kernel of known traffic: 2 GiB read + 1 GiB written), then one stage-1 and
-    three stage-2 launches."""
+    three stage-2 launches and one HCR-2 invocation (the two-iteration pair)."""
     import ctypes as C
     import torch
     p, s, G, u, v, w, dt = setup(ls, n, mode)
@@ -574,6 +574,7 @@ def profile_pass(ls, n, mode):
     s.rk3_stage(1, G, G, x, u, v, w, dt=dt)
     for _ in range(3):
         s.rk3_stage(2, x, G, y, u, v, w, dt=dt)
+    s.reinit(y)  # the HCR-2 pair kernel
     torch.cuda.synchronize()
     print("profile pass done", flush=True)
 

@@ -659,7 +659,13 @@ struct HGeom {
 // phi_n (P) and phi_0 (Q) at the cell and its neighbours in flag order
 // x-, x+, y-, y+, z-, z+ (reinitialization.h:46-89).  Crossing flags and r~
 // come from phi_0 with the arithmetic of computeInterfaceCrossings.
-__device__ __forceinline__ double hcr2_point(const HGeom& hg, const double (&P)[7], const double (&Q)[7]) {
+// The part of a point update that depends on phi_0 alone: crossing flags,
+// the smoothed sign S and r~.  Both iterations of an invocation share it.
+struct HQ {
+    double S, rt;
+    int flag;
+};
+__device__ __forceinline__ HQ hcr2_q(const HGeom& hg, const double (&Q)[7]) {
     const double pc = Q[0];
     double sum0 = 0.0;
     int flag = 0;
@@ -670,8 +676,19 @@ __device__ __forceinline__ double hcr2_point(const HGeom& hg, const double (&P)[
             flag |= 1 << q;
         }
     }
+    HQ h;
+    h.S = pc * rsqrt_nr(fma(pc, pc, hg.dx2));
+    h.flag = flag;
+    // r~ = phi_0 / sum of crossing neighbours, 0 when that sum vanishes
+    h.rt = (flag != 0 && fabs(sum0) > 1e-15) ? div_fast(pc, sum0) : 0.0;
+    return h;
+}
+
+// The rest of the update from phi_n (P), phi_0 (Q) and their shared part.
+__device__ __forceinline__ double hcr2_p(const HGeom& hg, const double (&P)[7], const double (&Q)[7], const HQ& h) {
+    const double pc = Q[0];
     const double phi = P[0];
-    const double S = pc * rsqrt_nr(fma(pc, pc, hg.dx2));
+    const double S = h.S;
     // Godunov |grad phi| (weno5.h:202-248): for S > 0 an axis contributes
     // max(max(D-, 0)^2, min(D+, 0)^2), for S <= 0 the mirrored min/max.  With
     // the sign of S folded into 1/h both read max(A, B)^2 over the
@@ -685,22 +702,24 @@ __device__ __forceinline__ double hcr2_point(const HGeom& hg, const double (&P)[
     const double s2 = (gx * gx + gy * gy) + gz * gz;
     const double term = -S * (sqrt(s2) - 1.0);
     double forcing = 0.0;
-    if (flag != 0) {
+    if (h.flag != 0) {
         bool stable = (P[0] > 0.0) == (pc > 0.0);
 #pragma unroll
         for (int q = 0; q < 6; q++)
-            if ((flag >> q) & 1) stable = stable && ((P[q + 1] > 0.0) == (Q[q + 1] > 0.0));
+            if ((h.flag >> q) & 1) stable = stable && ((P[q + 1] > 0.0) == (Q[q + 1] > 0.0));
         if (stable) {
             double sum = 0.0;
 #pragma unroll
             for (int q = 0; q < 6; q++)
-                if ((flag >> q) & 1) sum += P[q + 1];
-            // r~ = phi_0 / sum of crossing neighbours, 0 when that sum vanishes
-            const double rt = fabs(sum0) > 1e-15 ? div_fast(pc, sum0) : 0.0;
-            forcing = hg.beta * fma(rt, sum, -phi) * hg.invdx;
+                if ((h.flag >> q) & 1) sum += P[q + 1];
+            forcing = hg.beta * fma(h.rt, sum, -phi) * hg.invdx;
         }
     }
     return fma(hg.dtau, term + forcing, phi);
+}
+
+__device__ __forceinline__ double hcr2_point(const HGeom& hg, const double (&P)[7], const double (&Q)[7]) {
+    return hcr2_p(hg, P, Q, hcr2_q(hg, Q));
 }
 
 template <int W, int H>
@@ -801,6 +820,11 @@ __global__ __launch_bounds__(256) void k_hcr2_zmarch(HGeom hg, const double* __r
 // Tiles are 64x8 (8 waves): the first iteration then covers 66x10 cells per
 // 512 outputs, 1.29 per cell, against 66x6 per 256 (1.55) on 64x4 tiles
 // (1.53 against 1.66 ms per pair at 512^3; 64x16 tiles were slower).
+// The fast mode keeps, per thread, the phi_0-only part of its own cell's
+// update (HQ: the sign flags, S, the constraint divisor) from the first
+// iteration to the second, one plane later, and is held to 76 VGPRs so six
+// waves fit per SIMD (three blocks per CU, as the LDS allows): 1.88 against
+// 1.93 ms for ls_reinit_hcr2 at 512^3 (at 88 VGPRs and four waves, 1.97 ms).
 #ifndef LS_PAIR_ROWS
 #define LS_PAIR_ROWS 8
 #endif
@@ -808,10 +832,9 @@ constexpr int PY = LS_PAIR_ROWS;                               // tile rows of t
 constexpr int PQX = HX + 4, PQY = PY + 4;                      // phi_0 plane: tile + 2-cell ring
 constexpr int P1X = HX + 2, P1Y = PY + 2;                      // phi_1 plane: tile + 1-cell ring
 constexpr int kQItems = PQX * PQY, kQPer = (kQItems + HX * PY - 1) / (HX * PY);
-constexpr int kP1Items = P1X * P1Y, kP1Per = (kP1Items + HX * PY - 1) / (HX * PY);
 
 template <bool EXACT>
-__global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* __restrict__ p0,
+__global__ __launch_bounds__(HX * PY) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 6))) void k_hcr2_pair0(HGeom hg, const double* __restrict__ p0,
                                                        double* __restrict__ pnew) {
     __shared__ double sQ[5][PQY][PQX];   // phi_0 plane kk in slot (kk + 10) % 5 (kk >= -2)
     __shared__ double sP1[2][P1Y][P1X];  // phi_1 plane kk in slot kk & 1
@@ -859,14 +882,36 @@ __global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* 
         for (int a = 0; a < kQPer; a++)
             if (tid + a * HX * PY < kQItems) sQ[sl][qy[a]][qx[a]] = val[a];
     };
-    // phi_1 items: local (x, y) in [-1, HX] x [-1, PY], evaluated at (ex, ey)
-    // (sQ indices; the clamped cell on a zero-gradient axis)
-    int px[kP1Per], py[kP1Per], ex[kP1Per], ey[kP1Per];
+    // phi_1 items: pass 0 the thread's own cell (local x = lx, y = ly), pass 1
+    // the one-cell ring around the tile (kRingItems items, threads below that
+    // count); each evaluated at (ex, ey) (sQ indices; the clamped cell on a
+    // zero-gradient axis).  In the fast mode the part of the update that
+    // depends on phi_0 alone (HQ) is kept from pass 0 for the thread's own
+    // cell and reused by the second iteration there one plane later.
+    constexpr int kRingItems = 2 * P1X + 2 * PY;
+    static_assert(kRingItems <= HX * PY, "ring items per thread");
+    int px[2], py[2], ex[2], ey[2];
+    px[0] = lx + 1;
+    py[0] = ly + 1;
+    {
+        const int r = min(tid, kRingItems - 1);
+        if (r < P1X) {
+            px[1] = r;
+            py[1] = 0;
+        } else if (r < 2 * P1X) {
+            px[1] = r - P1X;
+            py[1] = PY + 1;
+        } else if (r < 2 * P1X + PY) {
+            px[1] = 0;
+            py[1] = r - 2 * P1X + 1;
+        } else {
+            px[1] = HX + 1;
+            py[1] = r - 2 * P1X - PY + 1;
+        }
+    }
+    const bool ring = tid < kRingItems;
 #pragma unroll
-    for (int a = 0; a < kP1Per; a++) {
-        const int e = min(tid + a * HX * PY, kP1Items - 1);
-        py[a] = e / P1X;
-        px[a] = e - py[a] * P1X;
+    for (int a = 0; a < 2; a++) {
         int gx = x0 - 1 + px[a], gy = y0 - 1 + py[a];
         if (zgx) gx = min(max(gx, 0), z.nx - 1);
         if (zgy) gy = min(max(gy, 0), z.ny - 1);
@@ -874,12 +919,37 @@ __global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* 
         ey[a] = gy - y0 + 2;
     }
     auto zev = [&](int kk) { return zgz ? min(max(kk, 0), z.nz - 1) : kk; };
+    // phi_0 at sQ position (X, Y) of plane kk and its six neighbours
+    auto qget = [&](int kk, int X, int Y, double (&Q)[7]) {
+        const int sc = slot(kk), sm = slot(kk - 1), sp = slot(kk + 1);
+        Q[0] = sQ[sc][Y][X];
+        Q[1] = sQ[sc][Y][X - 1];
+        Q[2] = sQ[sc][Y][X + 1];
+        Q[3] = sQ[sc][Y - 1][X];
+        Q[4] = sQ[sc][Y + 1][X];
+        Q[5] = sQ[sm][Y][X];
+        Q[6] = sQ[sp][Y][X];
+    };
     // phi_1 on plane kk at sQ position (X, Y): the first iteration, phi_n = phi_0
     auto phi1 = [&](int kk, int X, int Y) {
-        const int sc = slot(kk), sm = slot(kk - 1), sp = slot(kk + 1);
-        const double Q[7] = {sQ[sc][Y][X],     sQ[sc][Y][X - 1], sQ[sc][Y][X + 1], sQ[sc][Y - 1][X],
-                             sQ[sc][Y + 1][X], sQ[sm][Y][X],     sQ[sp][Y][X]};
+        double Q[7];
+        qget(kk, X, Y, Q);
         return point(Q, Q);
+    };
+    // phi_1 of the thread's own cell on plane kk, keeping the phi_0 part
+    HQ hq_cur, hq_next;
+    hq_cur.S = hq_cur.rt = hq_next.S = hq_next.rt = 0.0;
+    hq_cur.flag = hq_next.flag = 0;
+    auto phi1_own = [&](int kk, HQ& hq) {
+        double Q[7];
+        qget(kk, ex[0], ey[0], Q);
+        if constexpr (EXACT) {
+            (void)hq;
+            return point(Q, Q);
+        } else {
+            hq = hcr2_q(hg, Q);
+            return hcr2_p(hg, Q, Q, hq);
+        }
     };
 
     double val[kQPer];
@@ -888,17 +958,15 @@ __global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* 
         put_q(kk, val);
     }
     __syncthreads();
-#pragma unroll
-    for (int a = 0; a < kP1Per; a++)
-        if (tid + a * HX * PY < kP1Items) sP1[kb & 1][py[a]][px[a]] = phi1(zev(kb), ex[a], ey[a]);
+    sP1[kb & 1][py[0]][px[0]] = phi1_own(zev(kb), hq_cur);
+    if (ring) sP1[kb & 1][py[1]][px[1]] = phi1(zev(kb), ex[1], ey[1]);
     double fm = phi1(zev(kb - 1), lx + 2, ly + 2);  // phi_1 of this column one plane below
 
     for (int k = kb; k < ke; k++) {
         if (k + 1 < ke) load_q(k + 3, val);
         __syncthreads();  // phi_0 plane k+2 in place; phi_1 plane k-1 no longer read
-#pragma unroll
-        for (int a = 0; a < kP1Per; a++)
-            if (tid + a * HX * PY < kP1Items) sP1[(k + 1) & 1][py[a]][px[a]] = phi1(zev(k + 1), ex[a], ey[a]);
+        sP1[(k + 1) & 1][py[0]][px[0]] = phi1_own(zev(k + 1), hq_next);
+        if (ring) sP1[(k + 1) & 1][py[1]][px[1]] = phi1(zev(k + 1), ex[1], ey[1]);
         __syncthreads();  // phi_1 plane k+1 in place
         // phi_0 plane k+3 into the slot of plane k-2, last read in the previous
         // plane, before this plane's stores: vmcnt counts stores too, so a wait
@@ -914,16 +982,20 @@ __global__ __launch_bounds__(HX * PY) void k_hcr2_pair0(HGeom hg, const double* 
                                  sP1[s0][Y + 1][X],
                                  fm,
                                  sP1[(k + 1) & 1][Y][X]};
-            const int sc = slot(k), QX = lx + 2, QY = ly + 2;
-            const double Q[7] = {sQ[sc][QY][QX],     sQ[sc][QY][QX - 1], sQ[sc][QY][QX + 1], sQ[sc][QY - 1][QX],
-                                 sQ[sc][QY + 1][QX], sQ[slot(k - 1)][QY][QX], sQ[slot(k + 1)][QY][QX]};
-            const double o = point(P, Q);
+            double Q[7];
+            qget(k, lx + 2, ly + 2, Q);
+            // an active cell is interior, so its pass-0 phi_0 position was
+            // (lx + 2, ly + 2) on plane k: hq_cur is this cell's phi_0 part
+            double o;
+            if constexpr (EXACT) o = point(P, Q);
+            else o = hcr2_p(hg, P, Q, hq_cur);
             const int64_t c = (int64_t)(k + 3) * sz + (int64_t)(j + 3) * z.sy + (i + 3);
             pnew[c] = o;
             if (i < 3 || i >= z.nx - 3 || j < 3 || j >= z.ny - 3 || k < 3 || k >= z.nz - 3)
                 store_images(z, pnew, i, j, k, o);
             fm = fc;
         }
+        hq_cur = hq_next;
     }
 }
 
