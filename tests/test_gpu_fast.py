@@ -1,8 +1,8 @@
 """GPU parity of the fast mode.
 
 The fast mode evaluates the same scheme with reorganised arithmetic (product-
-form WENO weights, one refined reciprocal per cell, FMA), so it is held to the
-north-star tolerance rather than bit equality: 1e-12 relative L-infinity in G
+form WENO weights, one refined reciprocal per cell, FMA), so it is held to a
+tolerance rather than bit equality: 1e-12 relative L-infinity in G
 over the interior and 1e-12 (relative to the box length) in the interpolated
 zero-level-set position on every grid edge.  Reference values: the CPU oracle
 at sizes it finishes in seconds, and the exact mode (bitwise equal to the
