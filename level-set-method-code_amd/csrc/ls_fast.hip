@@ -667,20 +667,23 @@ struct HQ {
 };
 __device__ __forceinline__ HQ hcr2_q(const HGeom& hg, const double (&Q)[7]) {
     const double pc = Q[0];
-    double sum0 = 0.0;
     int flag = 0;
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
-        if (pc * Q[q + 1] < 0.0) {
-            sum0 += Q[q + 1];
-            flag |= 1 << q;
-        }
-    }
+    for (int q = 0; q < 6; q++)
+        if (pc * Q[q + 1] < 0.0) flag |= 1 << q;
     HQ h;
     h.S = pc * rsqrt_nr(fma(pc, pc, hg.dx2));
     h.flag = flag;
-    // r~ = phi_0 / sum of crossing neighbours, 0 when that sum vanishes
-    h.rt = (flag != 0 && fabs(sum0) > 1e-15) ? div_fast(pc, sum0) : 0.0;
+    h.rt = 0.0;
+    // r~ = phi_0 / sum of crossing neighbours, 0 when that sum vanishes.  Only
+    // cells beside the interface have a flag, so most wavefronts skip this.
+    if (flag != 0) {
+        double sum0 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+            if ((flag >> q) & 1) sum0 += Q[q + 1];
+        if (fabs(sum0) > 1e-15) h.rt = div_fast(pc, sum0);
+    }
     return h;
 }
 
@@ -691,14 +694,19 @@ __device__ __forceinline__ double hcr2_p(const HGeom& hg, const double (&P)[7], 
     const double S = h.S;
     // Godunov |grad phi| (weno5.h:202-248): for S > 0 an axis contributes
     // max(max(D-, 0)^2, min(D+, 0)^2), for S <= 0 the mirrored min/max.  With
-    // the sign of S folded into 1/h both read max(A, B)^2 over the
-    // non-negative A = max(+-D-, 0), B = max(-+D+, 0), which is the same value
-    // bit for bit (A^2 >= B^2 exactly when A >= B).
-    const bool pos = S > 0.0;
-    const double hx = pos ? hg.invdx : -hg.invdx, hy = pos ? hg.invdy : -hg.invdy, hz = pos ? hg.invdz : -hg.invdz;
-    const double gx = fmax(fmax((P[0] - P[1]) * hx, 0.0), fmax((P[0] - P[2]) * hx, 0.0));
-    const double gy = fmax(fmax((P[0] - P[3]) * hy, 0.0), fmax((P[0] - P[4]) * hy, 0.0));
-    const double gz = fmax(fmax((P[0] - P[5]) * hz, 0.0), fmax((P[0] - P[6]) * hz, 0.0));
+    // s = +-1 the sign of S, both read max(A, B)^2 over the non-negative
+    // A = max(s (phi_c - phi_lo) / h, 0), B = max(s (phi_c - phi_hi) / h, 0),
+    // the same value bit for bit (A^2 >= B^2 exactly when A >= B).  Since
+    // negation is exact and rounding is monotone, max(A, B) is
+    // max(RN(RN(s phi_c - min(s phi_lo, s phi_hi)) / h), 0) with the products
+    // by 1/h: one subtraction and one product per axis.
+    const int sgn = S > 0.0 ? 0 : (int)0x80000000u;  // the sign bit of -s
+    double R[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) R[q] = __hiloint2double(__double2hiint(P[q]) ^ sgn, __double2loint(P[q]));
+    const double gx = fmax((R[0] - fmin(R[1], R[2])) * hg.invdx, 0.0);
+    const double gy = fmax((R[0] - fmin(R[3], R[4])) * hg.invdy, 0.0);
+    const double gz = fmax((R[0] - fmin(R[5], R[6])) * hg.invdz, 0.0);
     const double s2 = (gx * gx + gy * gy) + gz * gz;
     const double term = -S * (sqrt(s2) - 1.0);
     double forcing = 0.0;
