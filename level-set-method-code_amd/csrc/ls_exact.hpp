@@ -172,42 +172,42 @@ __device__ __forceinline__ double hcr2_core(const double (&P)[7], const double (
                                             double dz, double dtau, double beta) {
     constexpr int NB = DIM == 3 ? 6 : 4;
     const double pc = Q[0];
-    double sum0 = 0.0;
-    int cnt = 0, flag = 0;
+    int flag = 0;
 #pragma unroll
-    for (int q = 0; q < NB; q++) {
-        const double pq = Q[q + 1];
-        if (pc * pq < 0.0) {
-            sum0 += pq;
-            cnt++;
-            flag |= 1 << q;
-        }
-    }
+    for (int q = 0; q < NB; q++)
+        if (pc * Q[q + 1] < 0.0) flag |= 1 << q;
+    // r~ (the reference sums the crossing neighbours in flag order); only cells
+    // beside the interface have a flag, so most wavefronts skip this
     double rt = 0.0;
-    if (cnt > 0 && fabs(sum0) > 1e-15) rt = pc / sum0;
+    if (flag != 0) {
+        double sum0 = 0.0;
+#pragma unroll
+        for (int q = 0; q < NB; q++)
+            if ((flag >> q) & 1) sum0 += Q[q + 1];
+        if (fabs(sum0) > 1e-15) rt = pc / sum0;
+    }
 
     const double phi = P[0];
     const double S = pc / sqrt(pc * pc + dx * dx);
-    double xm = (P[0] - P[1]) / dx, xp = (P[2] - P[0]) / dx;
-    double ym = (P[0] - P[3]) / dy, yp = (P[4] - P[0]) / dy;
-    double zm = 0.0, zp = 0.0;
+    // Godunov terms.  The reference takes, per axis, max(a^2, b^2) with
+    // a = max(D-, 0), b = min(D+, 0) for S > 0 and the mirrored min/max
+    // otherwise, D- = (phi_c - phi_lo) / h, D+ = (phi_hi - phi_c) / h.  With
+    // s = +-1 the sign of S that is max(A, B)^2 over A = max(s (phi_c - phi_lo)
+    // / h, 0), B = max(s (phi_c - phi_hi) / h, 0): negation is exact, and a
+    // rounded square, a rounded difference and a rounded quotient by h > 0
+    // are monotone, so max(A, B) = max(RN(RN(s phi_c - min(s phi_lo,
+    // s phi_hi)) / h), 0) and the axis costs one division instead of two.
+    const int sgn = S > 0.0 ? 0 : (int)0x80000000u;  // the sign bit of -s
+    double R[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) R[q] = __hiloint2double(__double2hiint(P[q]) ^ sgn, __double2loint(P[q]));
+    const double gx = fmax((R[0] - fmin(R[1], R[2])) / dx, 0.0);
+    const double gy = fmax((R[0] - fmin(R[3], R[4])) / dy, 0.0);
+    double s2 = gx * gx + gy * gy;
     if (DIM == 3) {
-        zm = (P[0] - P[5]) / dz;
-        zp = (P[6] - P[0]) / dz;
+        const double gz = fmax((R[0] - fmin(R[5], R[6])) / dz, 0.0);
+        s2 = s2 + gz * gz;
     }
-    double ax, bx, ay, by, az, bz;
-    if (S > 0.0) {
-        ax = fmax(xm, 0.0); bx = fmin(xp, 0.0);
-        ay = fmax(ym, 0.0); by = fmin(yp, 0.0);
-        az = fmax(zm, 0.0); bz = fmin(zp, 0.0);
-    } else {
-        ax = fmin(xm, 0.0); bx = fmax(xp, 0.0);
-        ay = fmin(ym, 0.0); by = fmax(yp, 0.0);
-        az = fmin(zm, 0.0); bz = fmax(zp, 0.0);
-    }
-    double s2;
-    if (DIM == 3) s2 = fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by) + fmax(az * az, bz * bz);
-    else s2 = fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by);
     const double term = -S * (sqrt(s2) - 1.0);
 
     double forcing = 0.0;
