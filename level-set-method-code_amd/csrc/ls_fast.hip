@@ -86,78 +86,6 @@ __global__ __launch_bounds__(256) void k_stage_ycell(YGrid m, int tiles_x, int64
     ycell<STAGE>(m, i, j, Gs, Gn, out, u, v, sl);
 }
 
-// HCR-2 pseudo-time step (reinitialization.h:136-193) with multiplies for the
-// divisions by dx and a refined reciprocal for r~ and the smoothed sign.
-template <int DIM>
-__global__ __launch_bounds__(256) void k_hcr2_fast(FastGeom f, const double* __restrict__ pn,
-                                                   const double* __restrict__ p0,
-                                                   double* __restrict__ pnew) {
-    const int64_t t = xcd_tile(blockIdx.x, f.nblocks);
-    if (t >= (int64_t)f.tiles_x * f.tiles_y * (DIM == 3 ? f.g.nz : 1)) return;
-    const int tx = (int)(t % f.tiles_x);
-    const int64_t r = t / f.tiles_x;
-    const int ty = (int)(r % f.tiles_y);
-    const int tz = (int)(r / f.tiles_y);
-    const int i = tx * kFX + (int)threadIdx.x;
-    const int j = ty * kFY + (int)threadIdx.y;
-    if (i >= f.g.nx || j >= f.g.ny) return;
-    const int k = DIM == 3 ? tz + f.g.ng : 0;
-    const int64_t c = (int64_t)k * f.g.sz + (int64_t)(j + f.g.ng) * f.g.sy + (i + f.g.ng);
-    const Geom& g = f.g;
-    const int64_t off[6] = {-1, 1, -g.sy, g.sy, -g.sz, g.sz};
-    constexpr int NB = DIM == 3 ? 6 : 4;
-    const double* P = pn + c;
-    const double* Q = p0 + c;
-    const double pc = Q[0];
-    double sum0 = 0.0;
-    int flag = 0;
-#pragma unroll
-    for (int q = 0; q < NB; q++) {
-        const double pq = Q[off[q]];
-        if (pc * pq < 0.0) {
-            sum0 += pq;
-            flag |= 1 << q;
-        }
-    }
-    const double phi = P[0];
-    const double S = pc * rsqrt_nr(fma(pc, pc, g.dx * g.dx));
-    const double xm = (P[0] - P[-1]) * f.invdx, xp = (P[1] - P[0]) * f.invdx;
-    const double ym = (P[0] - P[-g.sy]) * f.invdy, yp = (P[g.sy] - P[0]) * f.invdy;
-    double zm = 0.0, zp = 0.0;
-    if (DIM == 3) {
-        zm = (P[0] - P[-g.sz]) * f.invdz;
-        zp = (P[g.sz] - P[0]) * f.invdz;
-    }
-    const bool pos = S > 0.0;
-    const double ax = pos ? fmax(xm, 0.0) : fmin(xm, 0.0), bx = pos ? fmin(xp, 0.0) : fmax(xp, 0.0);
-    const double ay = pos ? fmax(ym, 0.0) : fmin(ym, 0.0), by = pos ? fmin(yp, 0.0) : fmax(yp, 0.0);
-    double s2 = fmax(ax * ax, bx * bx) + fmax(ay * ay, by * by);
-    if (DIM == 3) {
-        const double az = pos ? fmax(zm, 0.0) : fmin(zm, 0.0), bz = pos ? fmin(zp, 0.0) : fmax(zp, 0.0);
-        s2 += fmax(az * az, bz * bz);
-    }
-    const double term = -S * (sqrt(s2) - 1.0);
-    double forcing = 0.0;
-    if (flag != 0) {
-        bool stable = (P[0] > 0.0) == (pc > 0.0);
-#pragma unroll
-        for (int q = 0; q < NB; q++)
-            if ((flag >> q) & 1) stable = stable && ((P[off[q]] > 0.0) == (Q[off[q]] > 0.0));
-        if (stable) {
-            double sum = 0.0;
-#pragma unroll
-            for (int q = 0; q < NB; q++)
-                if ((flag >> q) & 1) sum += P[off[q]];
-            // r~ = phi_0 / sum of crossing neighbours, 0 when that sum vanishes
-            // (reinitialization.h:356-358)
-            const double rt = fabs(sum0) > 1e-15 ? div_fast(pc, sum0) : 0.0;
-            forcing = g.beta * fma(rt, sum, -phi) * f.invdx;
-        }
-    }
-    pnew[c] = fma(g.dtau, term + forcing, phi);
-}
-
-
 // ---------------------------------------------------------------------------
 // 3D kernels march along z: a block owns a column of cells of the x-y plane
 // and walks a chunk of z planes, keeping the column's z stencil in registers
@@ -665,11 +593,13 @@ struct HQ {
     double S, rt;
     int flag;
 };
-__device__ __forceinline__ HQ hcr2_q(const HGeom& hg, const double (&Q)[7]) {
+template <int DIM = 3, class H>
+__device__ __forceinline__ HQ hcr2_q(const H& hg, const double (&Q)[7]) {
+    constexpr int NB = DIM == 3 ? 6 : 4;
     const double pc = Q[0];
     int flag = 0;
 #pragma unroll
-    for (int q = 0; q < 6; q++)
+    for (int q = 0; q < NB; q++)
         if (pc * Q[q + 1] < 0.0) flag |= 1 << q;
     HQ h;
     h.S = pc * rsqrt_nr(fma(pc, pc, hg.dx2));
@@ -680,7 +610,7 @@ __device__ __forceinline__ HQ hcr2_q(const HGeom& hg, const double (&Q)[7]) {
     if (flag != 0) {
         double sum0 = 0.0;
 #pragma unroll
-        for (int q = 0; q < 6; q++)
+        for (int q = 0; q < NB; q++)
             if ((flag >> q) & 1) sum0 += Q[q + 1];
         if (fabs(sum0) > 1e-15) h.rt = div_fast(pc, sum0);
     }
@@ -688,7 +618,9 @@ __device__ __forceinline__ HQ hcr2_q(const HGeom& hg, const double (&Q)[7]) {
 }
 
 // The rest of the update from phi_n (P), phi_0 (Q) and their shared part.
-__device__ __forceinline__ double hcr2_p(const HGeom& hg, const double (&P)[7], const double (&Q)[7], const HQ& h) {
+template <int DIM = 3, class H>
+__device__ __forceinline__ double hcr2_p(const H& hg, const double (&P)[7], const double (&Q)[7], const HQ& h) {
+    constexpr int NB = DIM == 3 ? 6 : 4;
     const double pc = Q[0];
     const double phi = P[0];
     const double S = h.S;
@@ -706,19 +638,22 @@ __device__ __forceinline__ double hcr2_p(const HGeom& hg, const double (&P)[7], 
     for (int q = 0; q < 7; q++) R[q] = __hiloint2double(__double2hiint(P[q]) ^ sgn, __double2loint(P[q]));
     const double gx = fmax((R[0] - fmin(R[1], R[2])) * hg.invdx, 0.0);
     const double gy = fmax((R[0] - fmin(R[3], R[4])) * hg.invdy, 0.0);
-    const double gz = fmax((R[0] - fmin(R[5], R[6])) * hg.invdz, 0.0);
-    const double s2 = (gx * gx + gy * gy) + gz * gz;
+    double s2 = gx * gx + gy * gy;
+    if (DIM == 3) {
+        const double gz = fmax((R[0] - fmin(R[5], R[6])) * hg.invdz, 0.0);
+        s2 = s2 + gz * gz;
+    }
     const double term = -S * (sqrt(s2) - 1.0);
     double forcing = 0.0;
     if (h.flag != 0) {
         bool stable = (P[0] > 0.0) == (pc > 0.0);
 #pragma unroll
-        for (int q = 0; q < 6; q++)
+        for (int q = 0; q < NB; q++)
             if ((h.flag >> q) & 1) stable = stable && ((P[q + 1] > 0.0) == (Q[q + 1] > 0.0));
         if (stable) {
             double sum = 0.0;
 #pragma unroll
-            for (int q = 0; q < 6; q++)
+            for (int q = 0; q < NB; q++)
                 if ((h.flag >> q) & 1) sum += P[q + 1];
             forcing = hg.beta * fma(h.rt, sum, -phi) * hg.invdx;
         }
@@ -726,8 +661,38 @@ __device__ __forceinline__ double hcr2_p(const HGeom& hg, const double (&P)[7], 
     return fma(hg.dtau, term + forcing, phi);
 }
 
-__device__ __forceinline__ double hcr2_point(const HGeom& hg, const double (&P)[7], const double (&Q)[7]) {
-    return hcr2_p(hg, P, Q, hcr2_q(hg, Q));
+template <int DIM = 3, class H>
+__device__ __forceinline__ double hcr2_point(const H& hg, const double (&P)[7], const double (&Q)[7]) {
+    return hcr2_p<DIM>(hg, P, Q, hcr2_q<DIM>(hg, Q));
+}
+
+// The parameters a point update reads (the members of HGeom it uses).
+struct HPar {
+    double dx2, invdx, invdy, invdz, dtau, beta;
+};
+
+// 2D HCR-2 pseudo-time step, one thread per cell, with the 3D kernels' point
+// update (x-, x+, y-, y+ neighbours; entries 5 and 6 unused).
+__global__ __launch_bounds__(256) void k_hcr2_fast2(FastGeom f, HPar hp, const double* __restrict__ pn,
+                                                    const double* __restrict__ p0, double* __restrict__ pnew) {
+    const int64_t t = xcd_tile(blockIdx.x, f.nblocks);
+    if (t >= (int64_t)f.tiles_x * f.tiles_y) return;
+    const int i = (int)(t % f.tiles_x) * kFX + (int)threadIdx.x;
+    const int j = (int)(t / f.tiles_x) * kFY + (int)threadIdx.y;
+    if (i >= f.g.nx || j >= f.g.ny) return;
+    const Geom& g = f.g;
+    const int64_t c = (int64_t)(j + g.ng) * g.sy + (i + g.ng);
+    const int64_t off[4] = {-1, 1, -g.sy, g.sy};
+    double P[7], Q[7];
+    P[0] = pn[c];
+    Q[0] = p0[c];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        P[q + 1] = pn[c + off[q]];
+        Q[q + 1] = p0[c + off[q]];
+    }
+    P[5] = P[6] = Q[5] = Q[6] = 0.0;
+    pnew[c] = hcr2_point<2>(hp, P, Q);
 }
 
 template <int W, int H>
@@ -1259,8 +1224,15 @@ int launch_hcr2_fast(const Geom& g, const double* pn, const double* p0, double* 
     }
     const FastGeom f = make_fast(g);
     if (f.nblocks > 0x7fffffffLL) return lsi::set_error(LS_ERR_INVALID, "grid too large for one launch");
+    HPar hp;
+    hp.dx2 = g.dx * g.dx;
+    hp.invdx = f.invdx;
+    hp.invdy = f.invdy;
+    hp.invdz = 0.0;
+    hp.dtau = g.dtau;
+    hp.beta = g.beta;
     const dim3 grid((unsigned)f.nblocks), block(kFX, kFY, 1);
-    hipLaunchKernelGGL((k_hcr2_fast<2>), grid, block, 0, st, f, pn, p0, pnew);
+    hipLaunchKernelGGL(k_hcr2_fast2, grid, block, 0, st, f, hp, pn, p0, pnew);
     if (hipGetLastError() != hipSuccess) return lsi::set_error(LS_ERR_DEVICE, "fast HCR-2 launch failed");
     return LS_OK;
 }
