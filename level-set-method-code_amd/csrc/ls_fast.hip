@@ -12,7 +12,8 @@
 //    choice is a select of six stencil values, not a divergent branch;
 //  * nonlinear weights use the product form
 //        w_k = d_k * prod_{j != k} t_j,   t_j = (eps + beta_j)^2,
-//    which equals d_k/t_k / sum_j d_j/t_j without any division;
+//    which equals d_k/t_k / sum_j d_j/t_j without any division (the kernels
+//    hold 16 t_j, a power-of-two factor that cancels exactly: see tsq);
 //  * the two face values enter the derivative as one fraction
 //        (N_b D_a - N_a D_b) / (6 h D_a D_b),
 //    and all axes of a cell go over one denominator, so a cell costs a single
@@ -148,8 +149,8 @@ struct ZGeom {
 // fraction, so a cell costs a single refined reciprocal:
 //    rhs = -(sum_a (u_a / 6h_a) N_a prod_{b != a} D_b) / prod_a D_a,
 //    N_a = n_hi d_lo - n_lo d_hi,  D_a = d_hi d_lo.
-// Smoothness terms T(c) = (eps + beta)^2 of the sub-stencil centred at c do
-// not depend on the orientation and are computed once per point.
+// Smoothness terms T(c) = 16 (eps + beta)^2 of the sub-stencil centred at c
+// do not depend on the orientation and are computed once per point.
 // Ghost images of a freshly computed interior value (periodic / zero-gradient).
 __device__ __forceinline__ void store_images(const ZGeom& z, double* __restrict__ out, int i, int j, int k,
                                              double o) {

@@ -34,12 +34,15 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
     return fma(y, e, y);
 }
 
-// t = (eps + beta)^2 of the 3-point sub-stencil (a, b, c) (weno5.h:28-37).
+// t = 16 (eps + beta)^2 of the 3-point sub-stencil (a, b, c) (weno5.h:28-37),
+// as (4 eps + 4 beta)^2 with 4 beta = (13/3) d^2 + e^2.  The faces use the
+// T's only in ratios, and scaling by a power of two commutes with rounding,
+// so the weights are those of (eps + beta)^2 bit for bit, one product fewer.
 __device__ __forceinline__ double tsq(double a, double b, double c) {
     const double d = fma(-2.0, b, a + c);
     const double e = c - a;
-    const double beta = fma((13.0 / 12.0) * d, d, 0.25 * (e * e));
-    const double s = lsx::kWenoEps + beta;
+    const double beta4 = fma((13.0 / 3.0) * d, d, e * e);
+    const double s = 4.0 * lsx::kWenoEps + beta4;
     return s * s;
 }
 
