@@ -134,15 +134,24 @@ LS_CRDIV_FN double dq(double a, double b, double y) {
 // [2^-1000, 2^700]; the exact-mode uses stay far inside it (divisors
 // (eps + beta)^2 in [2^-40, 2^214], weight sums in [2^-218, 2^42], |grad G|
 // in [2^-100, 2^117]; numerators 0.1, 0.3, 0.6 and 1).  Device only.
-__device__ __forceinline__ double rdiv_u(double a, double b) {
+//
+// The sequence splits into a part that reads b alone (rcp_u, the refined
+// reciprocal) and the quotient from it (rdiv_y), so that several quotients by
+// one divisor can share the first part; rdiv_u is their composition.
+__device__ __forceinline__ double rcp_u(double b) {
     double y = __builtin_amdgcn_rcp(b);
     double e = fma(-b, y, 1.0);
     y = fma(y, e, y);
     e = fma(-b, y, 1.0);
-    y = fma(y, e, y);
+    return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rdiv_y(double a, double b, double y) {
     const double q = a * y;
     return fma(fma(b, q, -a), -y, q);
 }
+
+__device__ __forceinline__ double rdiv_u(double a, double b) { return rdiv_y(a, b, rcp_u(b)); }
 
 // a / b in the flavour D: the IEEE division, or rdiv_u.
 template <int D>

@@ -82,6 +82,53 @@ __device__ __forceinline__ double weno_r(double v0, double v1, double v2, double
     return w0 * p0 + w1 * p1 + w2 * p2;
 }
 
+#if defined(__HIPCC__)
+// The smoothness data of the 3-point sub-stencil centred at c in each
+// orientation, as weno_l (x: beta3(G(c-1), G(c), G(c+1))) and weno_r
+// (y: beta3(G(c+1), G(c), G(c-1))) compute it -- the two add the outer
+// values in opposite orders, so their bits may differ -- each as
+// (t, rcp_u(t)), t = (eps + beta)^2: the divisor of the weight numerator and
+// the reciprocal part of its rdiv_u.  A sub-stencil serves three faces of one
+// orientation, so the marching kernels compute this once per centre.
+struct Sub2 {
+    double2 l, r;
+};
+__device__ __forceinline__ Sub2 sub_terms(double gm, double g0, double gp) {
+    const double bl = beta3(gm, g0, gp), br = beta3(gp, g0, gm);
+    const double tl = (kWenoEps + bl) * (kWenoEps + bl), tr = (kWenoEps + br) * (kWenoEps + br);
+    Sub2 s;
+    s.l = make_double2(tl, rcp_u(tl));
+    s.r = make_double2(tr, rcp_u(tr));
+    return s;
+}
+
+// weno_l / weno_r with the FMA division sequences (kDivFma), from the
+// sub-stencil data (t, rcp_u(t)) of its three centres in the order weno_l /
+// weno_r takes its beta_0, beta_1, beta_2: the same operations on the same
+// values, so the same bits.
+template <bool MIRROR>
+__device__ __forceinline__ double weno_t(double v0, double v1, double v2, double v3, double v4, double2 s0,
+                                         double2 s1, double2 s2) {
+    double a0 = rdiv_y(MIRROR ? 0.3 : 0.1, s0.x, s0.y);
+    double a1 = rdiv_y(0.6, s1.x, s1.y);
+    double a2 = rdiv_y(MIRROR ? 0.1 : 0.3, s2.x, s2.y);
+    double s = a0 + a1 + a2;
+    const double ys = rdiv_u(1.0, s);
+    double w0 = qdiv_u(a0, s, ys), w1 = qdiv_u(a1, s, ys), w2 = qdiv_u(a2, s, ys);
+    double p0, p1, p2;
+    if (MIRROR) {
+        p0 = div6_u(11.0 * v2 - 7.0 * v3 + 2.0 * v4);
+        p1 = div6_u(2.0 * v1 + 5.0 * v2 - v3);
+        p2 = div6_u(-v0 + 5.0 * v1 + 2.0 * v2);
+    } else {
+        p0 = div6_u(2.0 * v0 - 7.0 * v1 + 11.0 * v2);
+        p1 = div6_u(-v1 + 5.0 * v2 + 2.0 * v3);
+        p2 = div6_u(2.0 * v2 + 5.0 * v3 - v4);
+    }
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+#endif
+
 // weno_l(u0..u4) when mirror is false; when mirror is true weno_r on the
 // stencil (u4, u3, u2, u1, u0), i.e. with its values given in mirrored order.
 // weno5_right is weno5_left's arithmetic on the mirrored stencil with the

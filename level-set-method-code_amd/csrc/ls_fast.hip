@@ -243,6 +243,9 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
     __shared__ double sG[FGH][FGW];                      // plane k: tile + 3-cell halo
     __shared__ double sTx[EXACT ? 1 : FY][FX + 4];       // T along x at local x = -2..FX+1
     __shared__ double sTy[EXACT ? 1 : FY + 4][FX];       // T along y at local y = -2..FY+1
+    // exact mode: sub-stencil data (t, rcp_u(t)) per centre, [left-biased, mirrored]
+    __shared__ double2 sSx[EXACT ? 2 : 1][EXACT ? FY : 1][EXACT ? FX + 4 : 1];  // local x = -2..FX+1
+    __shared__ double2 sSy[EXACT ? 2 : 1][EXACT ? FY + 4 : 1][EXACT ? FX : 1];  // local y = -2..FY+1
     __shared__ FV sFx[2][FY][FX + 1];                    // [left-biased, mirrored] x face between local x s-1 and s
     __shared__ FV sFy[2][FY + 1][FX];                    // [left-biased, mirrored] y face between local y s-1 and s
     __shared__ unsigned char sOri[FY][FX];     // bit 0: u_eff >= 0, bit 1: v_eff >= 0
@@ -380,6 +383,29 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         __syncthreads();  // plane k in sG
         // every value of plane k's stencils was checked before this barrier
         const bool fd = EXACT && z.crdiv && __builtin_amdgcn_readfirstlane(sBad) == 0;
+        // --- exact mode with the FMA sequences: each sub-stencil's (t, 1/t)
+        // once per centre, for the three x (y) faces of an orientation that use it
+        if constexpr (EXACT) {
+            if (fd) {
+                auto subx = [&](int row, int q) {  // centre sG[row + 3][q + 1], local x q - 2
+                    const lsx::Sub2 b = lsx::sub_terms(sG[row + 3][q], sG[row + 3][q + 1], sG[row + 3][q + 2]);
+                    sSx[0][row][q] = b.l;
+                    sSx[1][row][q] = b.r;
+                };
+                auto suby = [&](int r, int c) {  // centre sG[r + 1][c + 3], local y r - 2
+                    const lsx::Sub2 b = lsx::sub_terms(sG[r][c + 3], sG[r + 1][c + 3], sG[r + 2][c + 3]);
+                    sSy[0][r][c] = b.l;
+                    sSy[1][r][c] = b.r;
+                };
+                subx(ly, lx + 2);
+                suby(ly + 2, lx);
+                if (ly < 4) suby(ly < 2 ? ly : ly + FY, lx);  // local rows -2, -1, FY, FY+1
+                if (ly == kWaveTx && lx < 4 * FY) {           // local x -2, -1, FX, FX+1 of each row
+                    const int c4 = lx & 3;
+                    subx(lx >> 2, c4 < 2 ? c4 : c4 + FX);
+                }
+            }
+        }
         // --- smoothness terms along x and y, effective velocity, orientation
         if (!EXACT) {
         sTx[ly][lx + 2] = tsq(sG[ly + 3][lx + 2], sG[ly + 3][lx + 3], sG[ly + 3][lx + 4]);
@@ -427,20 +453,35 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         __syncthreads();  // T, orientations
         // --- faces
         // face c+1/2 of an axis into slot[0] (left-biased) and slot[OS] (mirrored)
-        auto put = [&](auto gs, auto ts, auto os, const double* g, const double* t, bool nL, bool nR, FV* slot) {
+        // t: the smoothness data at the face's low cell c (fast: T; exact: (t, 1/t)
+        // per orientation), TS its stride along the axis
+        auto put = [&](auto gs, auto ts, auto os, const double* g, auto t, bool nL, bool nR, FV* slot) {
             constexpr int GS = decltype(gs)::value, TS = decltype(ts)::value, OS = decltype(os)::value;
             if constexpr (EXACT) {
-                (void)t;
                 if (fd) {
-                    if (nL) slot[0] = xface_lds<GS, lsx::kDivFma>(g, false);
-                    if (nR) slot[OS] = xface_lds<GS, lsx::kDivFma>(g, true);
+                    // weno_l's beta_0..2 sit at centres c-1, c, c+1; weno_r's at c+2, c+1, c
+                    constexpr int SO = GS == 1 ? FY * (FX + 4) : (FY + 4) * FX;  // to the mirrored set
+                    if (nL)
+                        slot[0] = lsx::weno_t<false>(g[-2 * GS], g[-GS], g[0], g[GS], g[2 * GS], t[-TS], t[0], t[TS]);
+                    if (nR)
+                        slot[OS] = lsx::weno_t<true>(g[-GS], g[0], g[GS], g[2 * GS], g[3 * GS], t[SO + 2 * TS],
+                                                     t[SO + TS], t[SO]);
                 } else {
+                    (void)t;
                     if (nL) slot[0] = xface_lds<GS, lsx::kDivIEEE>(g, false);
                     if (nR) slot[OS] = xface_lds<GS, lsx::kDivIEEE>(g, true);
                 }
             } else {
                 face_put<GS, TS, OS>(g, t, nL, nR, slot);
             }
+        };
+        auto tbx = [&](int row, int q) {  // smoothness data along x at local x q - 2 of a row
+            if constexpr (EXACT) return &sSx[0][row][q];
+            else return &sTx[row][q];
+        };
+        auto tby = [&](int r, int c) {  // smoothness data along y at local y r - 2 of a column
+            if constexpr (EXACT) return &sSy[0][r][c];
+            else return &sTy[r][c];
         };
         using C1 = std::integral_constant<int, 1>;
         using CGW = std::integral_constant<int, FGW>;
@@ -458,8 +499,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
                     nL = nL || o;
                     nR = nR || !o;
                 }
-                put(C1(), C1(), COX(), g, &sTx[0][0] + (EXACT ? 0 : ly * (FX + 4) + lx + 2), nL, nR,
-                    &sFx[0][ly][lx + 1]);
+                put(C1(), C1(), COX(), g, tbx(ly, lx + 2), nL, nR, &sFx[0][ly][lx + 1]);
             }
             {  // y face j+1/2
                 bool nL = Ly, nR = !Ly;
@@ -468,8 +508,7 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
                     nL = nL || o;
                     nR = nR || !o;
                 }
-                put(CGW(), CFX(), COY(), g, &sTy[0][0] + (EXACT ? 0 : (ly + 2) * FX + lx), nL, nR,
-                    &sFy[0][ly + 1][lx]);
+                put(CGW(), CFX(), COY(), g, tby(ly + 2, lx), nL, nR, &sFy[0][ly + 1][lx]);
             }
             if constexpr (EXACT) {
                 // z faces as single values in zlo.n
@@ -503,14 +542,12 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         if (ly == kWaveFx && lx < FY) {  // x face on the low edge of row lx, in its tile cell's orientation
             if (FULL || y0 + lx < z.ny) {
                 const bool o = sOri[lx][0] & 1;
-                put(C1(), C1(), COX(), &sG[lx + 3][2], &sTx[0][0] + (EXACT ? 0 : lx * (FX + 4) + 1), o, !o,
-                    &sFx[0][lx][0]);
+                put(C1(), C1(), COX(), &sG[lx + 3][2], tbx(lx, 1), o, !o, &sFx[0][lx][0]);
             }
         } else if (ly == kWaveFy) {  // y face on the low edge of column lx
             if (FULL || x0 + lx < z.nx) {
                 const bool o = (sOri[0][lx] >> 1) & 1;
-                put(CGW(), CFX(), COY(), &sG[2][lx + 3], &sTy[0][0] + (EXACT ? 0 : FX + lx), o, !o,
-                    &sFy[0][0][lx]);
+                put(CGW(), CFX(), COY(), &sG[2][lx + 3], tby(1, lx), o, !o, &sFy[0][0][lx]);
             }
         }
         __syncthreads();  // faces
