@@ -90,17 +90,12 @@ __device__ __forceinline__ double weno_r(double v0, double v1, double v2, double
 // (t, rcp_u(t)), t = (eps + beta)^2: the divisor of the weight numerator and
 // the reciprocal part of its rdiv_u.  A sub-stencil serves three faces of one
 // orientation, so the marching kernels compute this once per centre.
-struct Sub2 {
-    double2 l, r;
-};
-__device__ __forceinline__ Sub2 sub_terms(double gm, double g0, double gp) {
-    const double bl = beta3(gm, g0, gp), br = beta3(gp, g0, gm);
-    const double tl = (kWenoEps + bl) * (kWenoEps + bl), tr = (kWenoEps + br) * (kWenoEps + br);
-    Sub2 s;
-    s.l = make_double2(tl, rcp_u(tl));
-    s.r = make_double2(tr, rcp_u(tr));
-    return s;
+__device__ __forceinline__ double2 sub_l(double gm, double g0, double gp) {
+    const double b = beta3(gm, g0, gp);
+    const double t = (kWenoEps + b) * (kWenoEps + b);
+    return make_double2(t, rcp_u(t));
 }
+__device__ __forceinline__ double2 sub_r(double gm, double g0, double gp) { return sub_l(gp, g0, gm); }
 
 // weno_l / weno_r with the FMA division sequences (kDivFma), from the
 // sub-stencil data (t, rcp_u(t)) of its three centres in the order weno_l /

@@ -385,24 +385,25 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         const bool fd = EXACT && z.crdiv && __builtin_amdgcn_readfirstlane(sBad) == 0;
         // --- exact mode with the FMA sequences: each sub-stencil's (t, 1/t)
         // once per centre, for the three x (y) faces of an orientation that use it
+        auto subx = [&](int row, int q, bool l, bool r) {  // centre sG[row + 3][q + 1], local x q - 2
+            if constexpr (EXACT) {
+                const double gm = sG[row + 3][q], g0 = sG[row + 3][q + 1], gp = sG[row + 3][q + 2];
+                if (l) sSx[0][row][q] = lsx::sub_l(gm, g0, gp);
+                if (r) sSx[1][row][q] = lsx::sub_r(gm, g0, gp);
+            }
+        };
         if constexpr (EXACT) {
             if (fd) {
-                auto subx = [&](int row, int q) {  // centre sG[row + 3][q + 1], local x q - 2
-                    const lsx::Sub2 b = lsx::sub_terms(sG[row + 3][q], sG[row + 3][q + 1], sG[row + 3][q + 2]);
-                    sSx[0][row][q] = b.l;
-                    sSx[1][row][q] = b.r;
-                };
                 auto suby = [&](int r, int c) {  // centre sG[r + 1][c + 3], local y r - 2
-                    const lsx::Sub2 b = lsx::sub_terms(sG[r][c + 3], sG[r + 1][c + 3], sG[r + 2][c + 3]);
-                    sSy[0][r][c] = b.l;
-                    sSy[1][r][c] = b.r;
+                    const double gm = sG[r][c + 3], g0 = sG[r + 1][c + 3], gp = sG[r + 2][c + 3];
+                    sSy[0][r][c] = lsx::sub_l(gm, g0, gp);
+                    sSy[1][r][c] = lsx::sub_r(gm, g0, gp);
                 };
-                subx(ly, lx + 2);
                 suby(ly + 2, lx);
                 if (ly < 4) suby(ly < 2 ? ly : ly + FY, lx);  // local rows -2, -1, FY, FY+1
                 if (ly == kWaveTx && lx < 4 * FY) {           // local x -2, -1, FX, FX+1 of each row
                     const int c4 = lx & 3;
-                    subx(lx >> 2, c4 < 2 ? c4 : c4 + FX);
+                    subx(lx >> 2, c4 < 2 ? c4 : c4 + FX, true, true);
                 }
             }
         }
@@ -450,6 +451,11 @@ __global__ __launch_bounds__(ZTile<EXACT>::kFaceThreads) __attribute__((amdgpu_w
         }
         const bool Lx = ue >= 0.0, Ly = ve >= 0.0, Lz = we >= 0.0;
         sOri[ly][lx] = (unsigned char)((Lx ? 1 : 0) | (Ly ? 2 : 0));
+        // the x sub-stencils of this row serve only the row's x faces, whose
+        // orientations are its cells' (the tile-edge ones are computed above
+        // in both), so a wavefront computes the sets its row needs
+        if (EXACT && fd)
+            subx(ly, lx + 2, __builtin_amdgcn_ballot_w64(Lx) != 0, __builtin_amdgcn_ballot_w64(!Lx) != 0);
         __syncthreads();  // T, orientations
         // --- faces
         // face c+1/2 of an axis into slot[0] (left-biased) and slot[OS] (mirrored)
