@@ -68,7 +68,7 @@ __global__ __launch_bounds__(lsf::kYW) __attribute__((amdgpu_waves_per_eu(MW ? M
 void k_batch_ymarch(const BMember* __restrict__ mem, const BTile* __restrict__ tiles, int ntiles, int ch,
                     const double* __restrict__ Gs, const double* __restrict__ Gn, double* __restrict__ out,
                     const double* __restrict__ u, const double* __restrict__ v) {
-    __shared__ lsf::YLds L;
+    __shared__ lsf::YLds<EXACT> L;
     // one unit per block, dealt XCD-contiguously (block b runs on XCD b % 8):
     // XCD e takes units [e*per, (e+1)*per), so the consecutive chunks of a
     // strip and its neighbour strips go through one L2

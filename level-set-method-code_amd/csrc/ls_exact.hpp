@@ -124,6 +124,24 @@ __device__ __forceinline__ double weno_t(double v0, double v1, double v2, double
 }
 #endif
 
+// The part of weno_u after the weight numerators a0..a2.
+template <int D>
+__device__ __forceinline__ double weno_u_tail(double u0, double u1, double u2, double u3, double u4, double a0,
+                                              double a1, double a2, bool mirror) {
+    double s = a0 + a1 + a2;
+    const double ys = D == kDivIEEE ? 0.0 : dr<D>(1.0, s);
+    double w0 = dq<D>(a0, s, ys), w1 = dq<D>(a1, s, ys), w2 = dq<D>(a2, s, ys);
+    // left: p0 = (2u0 - 7u1) + 11u2, p1 = (-u1 + 5u2) + 2u3, p2 = (2u2 + 5u3) - u4;
+    // right: the first and last term of each swapped
+    const double e0 = 2.0 * u0, f0 = 11.0 * u2;
+    const double e1 = -u1, f1 = 2.0 * u3;
+    const double e2 = 2.0 * u2, f2 = -u4;
+    double p0 = d6<D>((mirror ? f0 : e0) - 7.0 * u1 + (mirror ? e0 : f0));
+    double p1 = d6<D>((mirror ? f1 : e1) + 5.0 * u2 + (mirror ? e1 : f1));
+    double p2 = d6<D>((mirror ? f2 : e2) + 5.0 * u3 + (mirror ? e2 : f2));
+    return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
 // weno_l(u0..u4) when mirror is false; when mirror is true weno_r on the
 // stencil (u4, u3, u2, u1, u0), i.e. with its values given in mirrored order.
 // weno5_right is weno5_left's arithmetic on the mirrored stencil with the
@@ -140,19 +158,23 @@ __device__ __forceinline__ double weno_u(double u0, double u1, double u2, double
     double a0 = dr<D>(d0, (kWenoEps + b0) * (kWenoEps + b0));
     double a1 = dr<D>(0.6, (kWenoEps + b1) * (kWenoEps + b1));
     double a2 = dr<D>(d2, (kWenoEps + b2) * (kWenoEps + b2));
-    double s = a0 + a1 + a2;
-    const double ys = D == kDivIEEE ? 0.0 : dr<D>(1.0, s);
-    double w0 = dq<D>(a0, s, ys), w1 = dq<D>(a1, s, ys), w2 = dq<D>(a2, s, ys);
-    // left: p0 = (2u0 - 7u1) + 11u2, p1 = (-u1 + 5u2) + 2u3, p2 = (2u2 + 5u3) - u4;
-    // right: the first and last term of each swapped
-    const double e0 = 2.0 * u0, f0 = 11.0 * u2;
-    const double e1 = -u1, f1 = 2.0 * u3;
-    const double e2 = 2.0 * u2, f2 = -u4;
-    double p0 = d6<D>((mirror ? f0 : e0) - 7.0 * u1 + (mirror ? e0 : f0));
-    double p1 = d6<D>((mirror ? f1 : e1) + 5.0 * u2 + (mirror ? e1 : f1));
-    double p2 = d6<D>((mirror ? f2 : e2) + 5.0 * u3 + (mirror ? e2 : f2));
-    return w0 * p0 + w1 * p1 + w2 * p2;
+    return weno_u_tail<D>(u0, u1, u2, u3, u4, a0, a1, a2, mirror);
 }
+
+#if defined(__HIPCC__)
+// weno_u with the FMA sequences from the sub-stencil data (t, rcp_u(t)) of
+// beta3(u0, u1, u2), beta3(u1, u2, u3), beta3(u2, u3, u4): sub_l at the
+// centres u1, u2, u3 when mirror is false; when it is true the stencil is
+// given mirrored, and those are sub_r at the same centres.
+__device__ __forceinline__ double weno_tu(double u0, double u1, double u2, double u3, double u4, double2 s0,
+                                          double2 s1, double2 s2, bool mirror) {
+    const double d0 = mirror ? 0.3 : 0.1, d2 = mirror ? 0.1 : 0.3;
+    double a0 = rdiv_y(d0, s0.x, s0.y);
+    double a1 = rdiv_y(0.6, s1.x, s1.y);
+    double a2 = rdiv_y(d2, s2.x, s2.y);
+    return weno_u_tail<kDivFma>(u0, u1, u2, u3, u4, a0, a1, a2, mirror);
+}
+#endif
 
 // Upwinded WENO-5 derivative from the 7-point axis stencil m3..p3.
 __device__ __forceinline__ double dweno(double m3, double m2, double m1, double c0, double p1,

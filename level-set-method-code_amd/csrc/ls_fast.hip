@@ -60,11 +60,11 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 // 2D stage: one wavefront per (64-column strip, chunk of rows), y-marching
 // (ls_ymarch.hpp).  Units are dealt XCD-contiguously, strips fastest.
 template <int STAGE, bool EXACT>
-__global__ __launch_bounds__(kYW) void k_stage_ymarch(YGrid m, int strips, int ch, int64_t units, int64_t nblocks,
+__global__ __launch_bounds__(kYW) __attribute__((amdgpu_waves_per_eu(EXACT ? 4 : 1))) void k_stage_ymarch(YGrid m, int strips, int ch, int64_t units, int64_t nblocks,
                                                       const double* __restrict__ Gs, const double* __restrict__ Gn,
                                                       double* __restrict__ out, const double* __restrict__ u,
                                                       const double* __restrict__ v, const double* __restrict__ sl) {
-    __shared__ YLds L;
+    __shared__ YLds<EXACT> L;
     const int64_t t = xcd_tile(blockIdx.x, nblocks);
     if (t >= units) return;
     const int sx = (int)(t % strips), cy = (int)(t / strips);

@@ -69,11 +69,14 @@ struct YRow {
 };
 
 // LDS of one wavefront.
+template <bool EXACT>
 struct YLds {
-    double g[kYW + 8];           // G of the current row at x = x0 - 3 + p
-    double t[kYW + 8];           // smoothness term T at the same positions
-    double2 f[2][kYW + 1];       // x face x0 - 1/2 + q as (n, d): [0] left-biased, [1] mirrored
-    unsigned char ori[kYW];      // upwind orientation along x of each lane's cell
+    double g[kYW + 8];                    // G of the current row at x = x0 - 3 + p
+    double t[EXACT ? 1 : kYW + 8];        // fast: smoothness term T at the same positions
+    double2 s[EXACT ? 2 : 1][EXACT ? kYW + 8 : 1];  // exact: sub-stencil data (t, 1/t) there,
+                                                    // [0] left-biased, [1] mirrored (lsx::sub_l / sub_r)
+    double2 f[2][kYW + 1];                // x face x0 - 1/2 + q as (n, d): [0] left-biased, [1] mirrored
+    unsigned char ori[kYW];               // upwind orientation along x of each lane's cell
 };
 
 // Fast stage combine (rk3.h:103-150 with FMA).
@@ -175,7 +178,7 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
                                              const double* __restrict__ Gs, const double* __restrict__ Gn,
                                              double* __restrict__ out, const double* __restrict__ u,
                                              const double* __restrict__ v, const double* __restrict__ sl,
-                                             YLds& L) {
+                                             YLds<EXACT>& L) {
     const int l = (int)threadIdx.x;
     const int xe = min(x0 + kYW, m.nx);
     const int i = x0 + l;
@@ -264,7 +267,7 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         const YRow fresh = fetch(j + LS_YMARCH_PF, true);
         __syncthreads();  // row
         // --- smoothness terms along x, u_eff, orientations
-        if (!EXACT) {
+        if constexpr (!EXACT) {
             if (act) L.t[3 + l] = tsq(L.g[2 + l], L.g[3 + l], L.g[4 + l]);
             if (l < 4) {  // T at x0-2, x0-1, xe, xe+1
                 const int p = l < 2 ? 1 + l : xe - x0 + 1 + l;
@@ -292,6 +295,21 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         }
         const bool Lx = ue >= 0.0, Ly = ve >= 0.0;
         L.ori[l] = (unsigned char)Lx;
+        if constexpr (EXACT) {
+            // the x sub-stencils' (t, 1/t), once per centre for the three faces
+            // of an orientation that use them; the strip's x faces take the
+            // orientations of its cells, so only the sets those use
+            if (fd) {
+                const bool wl = __ballot(act && Lx) != 0, wr = __ballot(act && !Lx) != 0;
+                auto sub = [&](int p) {  // centre L.g[p]
+                    const double gm = L.g[p - 1], g0 = L.g[p], gp = L.g[p + 1];
+                    if (wl) L.s[0][p] = lsx::sub_l(gm, g0, gp);
+                    if (wr) L.s[1][p] = lsx::sub_r(gm, g0, gp);
+                };
+                if (act) sub(3 + l);
+                if (l < 4) sub(l < 2 ? 1 + l : xe - x0 + 1 + l);  // centres x0-2, x0-1, xe, xe+1
+            }
+        }
         __syncthreads();  // T, orientations
         // --- faces
         Face yhi;
@@ -304,7 +322,7 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
                 nL = nL || o;
                 nR = nR || !o;
             }
-            if (EXACT) {  // weno5_left / weno5_right values (weno5.h:62-122); .x holds the value
+            if constexpr (EXACT) {  // weno5_left / weno5_right values (weno5.h:62-122); .x holds the value
                 // orientations are per-lane choices of the stencil order
                 // (lsx::weno_u), not branches: every lane builds the first x
                 // face it needs, lanes that need both then the mirrored one
@@ -314,7 +332,15 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
                     auto xf = [&](bool mir) {  // weno_l(g[-2..2]) or weno_r(g[-1..3])
                         const double* b = mir ? g + 3 : g - 2;
                         const int st = mir ? -1 : 1;
-                        return lsx::weno_u<D>(b[0], b[st], b[2 * st], b[3 * st], b[4 * st], mir);
+                        if constexpr (D == lsx::kDivFma) {
+                            // beta_0..2 at the centres g[-1], g[0], g[1] (left-biased set)
+                            // or g[2], g[1], g[0] (mirrored set)
+                            const double2* sb = mir ? &L.s[1][3 + l + 2] : &L.s[0][3 + l - 1];
+                            return lsx::weno_tu(b[0], b[st], b[2 * st], b[3 * st], b[4 * st], sb[0], sb[st],
+                                                sb[2 * st], mir);
+                        } else {
+                            return lsx::weno_u<D>(b[0], b[st], b[2 * st], b[3 * st], b[4 * st], mir);
+                        }
                     };
                     const bool m1 = !nL;
                     L.f[m1 ? 1 : 0][l + 1].x = xf(m1);
@@ -341,14 +367,18 @@ __device__ __forceinline__ void ymarch_strip(const YGrid& m, int x0, int y0, int
         }
         if (l == 0) {  // x face x0-1/2 in the orientation of the strip's first cell
             const bool m = !Lx;  // weno_l(L.g[0..4]) or weno_r(L.g[1..5])
-            if (EXACT && fd)
-                L.f[m ? 1 : 0][0].x = lsx::weno_u<lsx::kDivFma>(L.g[m ? 5 : 0], L.g[m ? 4 : 1], L.g[m ? 3 : 2],
-                                                                L.g[m ? 2 : 3], L.g[m ? 1 : 4], m);
-            else if (EXACT)
-                L.f[m ? 1 : 0][0].x = lsx::weno_u<lsx::kDivIEEE>(L.g[m ? 5 : 0], L.g[m ? 4 : 1], L.g[m ? 3 : 2],
-                                                                 L.g[m ? 2 : 3], L.g[m ? 1 : 4], m);
-            else
+            if constexpr (EXACT) {
+                if (fd) {  // centres L.g[1..3] (left-biased set) or L.g[4..2] (mirrored set)
+                    const double2* sb = m ? &L.s[1][4] : &L.s[0][1];
+                    const int st = m ? -1 : 1;
+                    L.f[m ? 1 : 0][0].x = lsx::weno_tu(L.g[m ? 5 : 0], L.g[m ? 4 : 1], L.g[m ? 3 : 2], L.g[m ? 2 : 3],
+                                                       L.g[m ? 1 : 4], sb[0], sb[st], sb[2 * st], m);
+                } else
+                    L.f[m ? 1 : 0][0].x = lsx::weno_u<lsx::kDivIEEE>(L.g[m ? 5 : 0], L.g[m ? 4 : 1], L.g[m ? 3 : 2],
+                                                                     L.g[m ? 2 : 3], L.g[m ? 1 : 4], m);
+            } else {
                 face_put<1, 1, kYW + 1>(&L.g[2], &L.t[2], Lx, !Lx, &L.f[0][0]);
+            }
         }
         __syncthreads();  // faces
         // --- derivatives over one denominator, RHS, stage combine
