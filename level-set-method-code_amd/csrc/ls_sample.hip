@@ -55,6 +55,17 @@ __device__ __forceinline__ double sbeta(double f0, double f1, double f2) {
     return (13.0 / 12.0) * (p * p) + (1.0 / 4.0) * (q * q);
 }
 
+// The part of sface after the weight numerators a0..a2.
+template <int D>
+__device__ __forceinline__ double sface_tail(double v0, double v1, double v2, double v3, double v4, double a0,
+                                             double a1, double a2) {
+    const double sum = a0 + a1 + a2;
+    const double ys = D == lsx::kDivIEEE ? 0.0 : lsx::dr<D>(1.0, sum);
+    const double w0 = lsx::dq<D>(a0, sum, ys), w1 = lsx::dq<D>(a1, sum, ys), w2 = lsx::dq<D>(a2, sum, ys);
+    return lsx::d6<D>(w0 * (2.0 * v0 - 7.0 * v1 + 11.0 * v2)) + lsx::d6<D>(w1 * (-v1 + 5.0 * v2 + 2.0 * v3)) +
+           lsx::d6<D>(w2 * (2.0 * v2 + 5.0 * v3 - v4));
+}
+
 // weno5_left of the reference (sample/weno5_rk3.cpp:17-32); D is the division
 // flavour (ls_crdiv.hpp), every choice giving the same bits.
 template <int D = lsx::kDivIEEE>
@@ -65,11 +76,26 @@ __device__ __forceinline__ double sface(double v0, double v1, double v2, double 
     const double a0 = lsx::dr<D>(0.1, (kWenoEps + b0) * (kWenoEps + b0));
     const double a1 = lsx::dr<D>(0.6, (kWenoEps + b1) * (kWenoEps + b1));
     const double a2 = lsx::dr<D>(0.3, (kWenoEps + b2) * (kWenoEps + b2));
-    const double sum = a0 + a1 + a2;
-    const double ys = D == lsx::kDivIEEE ? 0.0 : lsx::dr<D>(1.0, sum);
-    const double w0 = lsx::dq<D>(a0, sum, ys), w1 = lsx::dq<D>(a1, sum, ys), w2 = lsx::dq<D>(a2, sum, ys);
-    return lsx::d6<D>(w0 * (2.0 * v0 - 7.0 * v1 + 11.0 * v2)) + lsx::d6<D>(w1 * (-v1 + 5.0 * v2 + 2.0 * v3)) +
-           lsx::d6<D>(w2 * (2.0 * v2 + 5.0 * v3 - v4));
+    return sface_tail<D>(v0, v1, v2, v3, v4, a0, a1, a2);
+}
+
+// The sub-stencil centred at f1: t = (eps + beta)^2, the divisor of its weight
+// numerator, and rcp_u(t), the part of that quotient's rdiv_u that reads t
+// alone.  A sub-stencil serves three faces, so the march computes this once.
+__device__ __forceinline__ double2 ssub(double f0, double f1, double f2) {
+    const double b = sbeta(f0, f1, f2);
+    const double t = (kWenoEps + b) * (kWenoEps + b);
+    return make_double2(t, lsx::rcp_u(t));
+}
+
+// sface<kDivFma> from the sub-stencil data of its centres v1, v2, v3: the same
+// operations on the same values, so the same bits.
+__device__ __forceinline__ double sface_t(double v0, double v1, double v2, double v3, double v4, double2 s0,
+                                          double2 s1, double2 s2) {
+    const double a0 = lsx::rdiv_y(0.1, s0.x, s0.y);
+    const double a1 = lsx::rdiv_y(0.6, s1.x, s1.y);
+    const double a2 = lsx::rdiv_y(0.3, s2.x, s2.y);
+    return sface_tail<lsx::kDivFma>(v0, v1, v2, v3, v4, a0, a1, a2);
 }
 
 // compute_geqn_rhs at one interior cell (sample/weno5_rk3.cpp:111-143) from
@@ -147,8 +173,9 @@ constexpr int kSW = 62;
 #endif
 
 struct SLds {
-    double g[72];  // G of the current row at padded column xs + p
-    double f[64];  // lane l's x face
+    double g[72];    // G of the current row at padded column xs + p
+    double f[64];    // lane l's x face
+    double2 c[68];   // FMA rows: ssub of the centre g[p]
 };
 
 template <int STAGE>
@@ -181,9 +208,16 @@ __global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int n
     for (int q = 0; q < 7; q++) bad |= !lsx::crdiv_safe(ring[q]);
     const bool fd_ok = g.crdiv && lsx::crdiv_safe(S_L);
     double fy0, fy1;  // y faces r-1/2, r+1/2
+    // FMA rows: ssub of the centres ring[3], ring[4] (the y face r+3/2 of the
+    // first row needs them, and each row passes them on).  The FMA rows are a
+    // prefix of the march (bad is sticky), so what a row carries was computed
+    // by FMA rows.
+    double2 cy0 = make_double2(0.0, 0.0), cy1 = cy0;
     if (fd_ok && __ballot(bad) == 0) {
         fy0 = sface<lsx::kDivFma>(ring[0], ring[1], ring[2], ring[3], ring[4]);
         fy1 = sface<lsx::kDivFma>(ring[1], ring[2], ring[3], ring[4], ring[5]);
+        cy0 = ssub(ring[2], ring[3], ring[4]);
+        cy1 = ssub(ring[3], ring[4], ring[5]);
     } else {
         fy0 = sface<lsx::kDivIEEE>(ring[0], ring[1], ring[2], ring[3], ring[4]);
         fy1 = sface<lsx::kDivIEEE>(ring[1], ring[2], ring[3], ring[4], ring[5]);
@@ -215,9 +249,17 @@ __global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int n
         }
         __syncthreads();  // row in LDS
         double fy2;  // y face r+3/2
-        if (fd) {
-            L.f[l] = sface<lsx::kDivFma>(L.g[l], L.g[l + 1], L.g[l + 2], L.g[l + 3], L.g[l + 4]);
-            fy2 = sface<lsx::kDivFma>(ring[2], ring[3], ring[4], ring[5], ring[6]);
+        double2 cy2 = make_double2(0.0, 0.0);  // FMA rows: ssub of the centre ring[5]
+        if (fd) {  // (wavefront-uniform; the block is one wavefront)
+            L.c[l + 2] = ssub(L.g[l + 1], L.g[l + 2], L.g[l + 3]);
+            if (l < 2) {  // centres g[1] and g[66]
+                const int p = l == 0 ? 1 : 66;
+                L.c[p] = ssub(L.g[p - 1], L.g[p], L.g[p + 1]);
+            }
+            cy2 = ssub(ring[4], ring[5], ring[6]);
+            fy2 = sface_t(ring[2], ring[3], ring[4], ring[5], ring[6], cy0, cy1, cy2);
+            __syncthreads();  // centre data
+            L.f[l] = sface_t(L.g[l], L.g[l + 1], L.g[l + 2], L.g[l + 3], L.g[l + 4], L.c[l + 1], L.c[l + 2], L.c[l + 3]);
         } else {
             L.f[l] = sface<lsx::kDivIEEE>(L.g[l], L.g[l + 1], L.g[l + 2], L.g[l + 3], L.g[l + 4]);
             fy2 = sface<lsx::kDivIEEE>(ring[2], ring[3], ring[4], ring[5], ring[6]);
@@ -247,6 +289,8 @@ __global__ __launch_bounds__(64) void k_sample_ymarch(SGeom g, int strips, int n
         bad |= !lsx::crdiv_safe(nr);
         fy0 = fy1;
         fy1 = fy2;
+        cy0 = cy1;
+        cy1 = cy2;
         cu = nu;
         cv = nv;
         cgn = ngn;
