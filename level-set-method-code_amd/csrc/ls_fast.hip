@@ -687,7 +687,9 @@ __device__ __forceinline__ double hcr2_p(const H& hg, const double (&P)[7], cons
         const double gz = fmax((R[0] - fmin(R[5], R[6])) * hg.invdz, 0.0);
         s2 = s2 + gz * gz;
     }
-    const double term = -S * (sqrt(s2) - 1.0);
+    // sqrt(s2) - 1 through sqrt_u on max(s2, 2^-767): below that bound both
+    // square roots are under 2^-54, so the difference rounds to -1 either way
+    const double term = -S * (sqrt_u(fmax(s2, 0x1p-767)) - 1.0);
     double forcing = 0.0;
     if (h.flag != 0) {
         bool stable = (P[0] > 0.0) == (pc > 0.0);
