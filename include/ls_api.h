@@ -206,17 +206,21 @@ int ls_check_finite(ls_solver_t* s, const double* G, void* stream);
  * two host threads) race on it.  Rejects the parameters ls_create rejects. */
 int ls_diag_noalloc(const ls_params_t* p, const double* G, const double* G0, double* out, void* stream);
 
-/* Device check of the exact mode's division sequences (no counterpart in the
- * reference; the tests call it): out[i] = a[i] / b[i] as the exact-mode
- * kernels compute it on the GPU, for device arrays of n doubles.
+/* Device check of the division and square-root sequences (no counterpart in
+ * the reference; the tests call it): out[i] = a[i] / b[i] (kinds 0-2) as the
+ * exact-mode kernels compute it on the GPU, for device arrays of n doubles.
  *   kind 0: rdiv_u, the sequence used for quotients with a divisor of their
  *           own (b in [2^-1000, 2^1000], a = 0 or |a| in [2^-960, 2^960],
  *           |a|/b in [2^-1000, 2^700]);
  *   kind 1: qdiv_u with y = rdiv_u(1, b), the sequence of quotients sharing a
  *           divisor (b in [2^-300, 2^300], a = 0 or |a| in [2^-960, 2^960],
  *           |a|/b in [2^-1000, 2^1000]);
- *   kind 2: div6_u, a / 6 (b unused; a = 0 or |a| in [2^-960, 2^1000]).
- * Inside those ranges each must equal the IEEE quotient bit for bit.
+ *   kind 2: div6_u, a / 6 (b unused; a = 0 or |a| in [2^-960, 2^1000]);
+ *   kind 3: sqrt_u, the square-root sequence of the fast HCR-2 (b unused;
+ *           a in [2^-767, 2^1000]): out[i] = sqrt(a[i]);
+ *   kind 4: sqrt_u(max(a, 2^-767)) - 1, the fast HCR-2's |grad phi| - 1 (b
+ *           unused; a = 0 or in (0, 2^1000]): out[i] = sqrt(a[i]) - 1.
+ * Inside those ranges each must equal the IEEE result bit for bit.
  * Stream-ordered, no synchronisation. */
 int ls_div_check(const double* a, const double* b, double* out, int64_t n, int32_t kind, void* stream);
 

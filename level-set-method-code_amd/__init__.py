@@ -312,10 +312,10 @@ def diag_noalloc(p, G, G0=None, out=None, stream=None):
 
 
 def div_check(a, b, kind, stream=None):
-    """ls_div_check: the exact mode's division sequence ``kind`` (0 rdiv_u,
-    1 qdiv_u on the reciprocal rdiv_u(1, b), 2 div6_u, which ignores b) on
-    contiguous float64 device tensors; returns the quotients as a new device
-    tensor."""
+    """ls_div_check: the device sequence ``kind`` (0 rdiv_u, 1 qdiv_u on the
+    reciprocal rdiv_u(1, b), 2 div6_u; 3 sqrt_u and 4 sqrt_u(max(a, 2^-767)) - 1,
+    the fast HCR-2's square root; kinds 2-4 ignore b) on contiguous float64
+    device tensors; returns the results as a new device tensor."""
     import torch
     for t in (a, b):
         if t is not None and (not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous()):

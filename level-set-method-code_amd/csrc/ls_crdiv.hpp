@@ -153,6 +153,23 @@ __device__ __forceinline__ double rdiv_y(double a, double b, double y) {
 
 __device__ __forceinline__ double rdiv_u(double a, double b) { return rdiv_y(a, b, rcp_u(b)); }
 
+// sqrt(x) for x in [2^-767, 2^1000]: v_rsq_f64 and the Goldschmidt
+// iteration with two Newton corrections that the compiler emits for sqrt on
+// gfx950, without the scaling of arguments below 2^-767 and the fix-up for 0
+// and inf around it.  Inside that range the two run the same instructions, so
+// this returns sqrt's bits.  Device only.
+__device__ __forceinline__ double sqrt_u(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    double d = fma(-g, g, x);
+    h = fma(h, r, h);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
 // a / b in the flavour D: the IEEE division, or rdiv_u.
 template <int D>
 __device__ __forceinline__ double dr(double a, double b) {

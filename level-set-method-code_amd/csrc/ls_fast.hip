@@ -689,7 +689,7 @@ __device__ __forceinline__ double hcr2_p(const H& hg, const double (&P)[7], cons
     }
     // sqrt(s2) - 1 through sqrt_u on max(s2, 2^-767): below that bound both
     // square roots are under 2^-54, so the difference rounds to -1 either way
-    const double term = -S * (sqrt_u(fmax(s2, 0x1p-767)) - 1.0);
+    const double term = -S * (lsx::sqrt_u(fmax(s2, 0x1p-767)) - 1.0);
     double forcing = 0.0;
     if (h.flag != 0) {
         bool stable = (P[0] > 0.0) == (pc > 0.0);

@@ -34,23 +34,6 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
     return fma(y, e, y);
 }
 
-// sqrt(x) for x in [2^-767, 2^1000]: v_rsq_f64 and the Goldschmidt
-// iteration with two Newton corrections that the compiler emits for sqrt on
-// gfx950, without the scaling of tiny arguments and the fix-up for 0 and inf
-// around it.  Inside that range the two run the same instructions, so this
-// returns sqrt's bits.
-__device__ __forceinline__ double sqrt_u(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = y * 0.5;
-    const double r = fma(-h, g, 0.5);
-    g = fma(g, r, g);
-    double d = fma(-g, g, x);
-    h = fma(h, r, h);
-    g = fma(d, h, g);
-    d = fma(-g, g, x);
-    return fma(d, h, g);
-}
-
 // t = 16 (eps + beta)^2 of the 3-point sub-stencil (a, b, c) (weno5.h:28-37),
 // as (4 eps + 4 beta)^2 with 4 beta = (13/3) d^2 + e^2.  The faces use the
 // T's only in ratios, and scaling by a power of two commutes with rounding,

@@ -380,7 +380,9 @@ __global__ void k_div_check(const double* __restrict__ a, const double* __restri
     double q;
     if (kind == 0) q = lsx::rdiv_u(x, b[i]);
     else if (kind == 1) q = lsx::qdiv_u(x, b[i], lsx::rdiv_u(1.0, b[i]));
-    else q = lsx::div6_u(x);
+    else if (kind == 2) q = lsx::div6_u(x);
+    else if (kind == 3) q = lsx::sqrt_u(x);
+    else q = lsx::sqrt_u(fmax(x, 0x1p-767)) - 1.0;
     out[i] = q;
 }
 
@@ -770,8 +772,8 @@ int ls_diag_noalloc(const ls_params_t* p, const double* G, const double* G0, dou
 }
 
 int ls_div_check(const double* a, const double* b, double* out, int64_t n, int32_t kind, void* stream) {
-    if (!a || !out || (kind != 2 && !b)) return fail(LS_ERR_INVALID, "NULL argument");
-    if (n < 0 || kind < 0 || kind > 2) return fail(LS_ERR_INVALID, "bad count or kind");
+    if (!a || !out || (kind < 2 && !b)) return fail(LS_ERR_INVALID, "NULL argument");
+    if (n < 0 || kind < 0 || kind > 4) return fail(LS_ERR_INVALID, "bad count or kind");
     if (n == 0) return LS_OK;
     const int64_t blocks = (n + 255) / 256;
     if (blocks > 0x7fffffffLL) return fail(LS_ERR_INVALID, "count too large for one launch");

@@ -1,4 +1,5 @@
-"""The exact mode's division sequences, run on the GPU, against IEEE division.
+"""The exact mode's division sequences and the fast HCR-2's square root, run
+on the GPU, against IEEE division and sqrt.
 
 The exact-mode kernels replace the reference's divisions
 (Local_code/level-set_MPI_3D/include/weno5.h:39-81, rk3.h:31-81) by three
@@ -111,6 +112,33 @@ def test_div6_u_is_ieee_division(ls, cuda):
     _check(got, a / 6.0, a, None)
 
 
+def test_sqrt_u_is_ieee_sqrt(ls, cuda):
+    """sqrt_u (the fast HCR-2's square root: the compiler's sequence without
+    its range scaling and fix-up) against the host's IEEE sqrt on
+    [2^-767, 2^1000], including both ends and the squared-gradient sums a
+    Godunov |grad phi| produces (sums of three squares of differences / h)."""
+    rng = np.random.default_rng(14)
+    a = _mag(rng, -767, 1000, N)
+    d = rng.standard_normal((3, N // 4)) * np.ldexp(1.0, rng.integers(-30, 30, (3, N // 4)))
+    sums = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+    sums = sums[sums >= 2.0 ** -767]
+    a = np.concatenate([a, sums, [2.0 ** -767, 2.0 ** 1000, 1.0, 2.0, 4.0, np.nextafter(1.0, 2.0)]])
+    got = _run(ls, cuda, a, None, 3)
+    _check(got, np.sqrt(a), a, None)
+
+
+def test_sqrt_u_term_is_ieee_below_the_range(ls, cuda):
+    """sqrt_u(max(s2, 2^-767)) - 1 equals sqrt(s2) - 1 for every s2 in
+    [0, 2^1000]: below 2^-767 both square roots are under 2^-54, so the
+    difference rounds to -1."""
+    rng = np.random.default_rng(15)
+    lo = _mag(rng, -1074, -767, N // 4)
+    hi = _mag(rng, -767, 1000, N // 4)
+    a = np.concatenate([lo, hi, [0.0, 5e-324, 2.0 ** -767, np.nextafter(2.0 ** -767, 0.0), 1.0, 2.0 ** -108]])
+    got = _run(ls, cuda, a, None, 4)
+    _check(got, np.sqrt(a) - 1.0, a, None)
+
+
 def test_div_check_rejects_bad_arguments(ls, cuda):
     import ctypes
     lib = ls.lib()
@@ -118,6 +146,8 @@ def test_div_check_rejects_bad_arguments(ls, cuda):
     x = cuda.zeros(4, dtype=cuda.float64, device="cuda")
     p = ctypes.c_void_p(x.data_ptr())
     assert lib.ls_div_check(p, p, p, -1, 0, None) != 0
-    assert lib.ls_div_check(p, p, p, 4, 3, None) != 0
+    assert lib.ls_div_check(p, p, p, 4, 5, None) != 0
+    assert lib.ls_div_check(p, None, p, 4, 1, None) != 0
     assert lib.ls_div_check(p, None, p, 4, 2, None) == 0
+    assert lib.ls_div_check(p, None, p, 4, 3, None) == 0
     cuda.cuda.synchronize()
