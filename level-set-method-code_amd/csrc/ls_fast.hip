@@ -58,12 +58,13 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
 }
 
 // 2D stage: one wavefront per (64-column strip, chunk of rows), y-marching
-// (ls_ymarch.hpp).  Units are dealt XCD-contiguously, strips fastest.
+// (ls_ymarch.hpp).  Units are dealt XCD-contiguously, strips fastest.  The
+// exact variant is held to four waves per SIMD (127 VGPRs).
 template <int STAGE, bool EXACT>
-__global__ __launch_bounds__(kYW) __attribute__((amdgpu_waves_per_eu(EXACT ? 4 : 1))) void k_stage_ymarch(YGrid m, int strips, int ch, int64_t units, int64_t nblocks,
-                                                      const double* __restrict__ Gs, const double* __restrict__ Gn,
-                                                      double* __restrict__ out, const double* __restrict__ u,
-                                                      const double* __restrict__ v, const double* __restrict__ sl) {
+__global__ __launch_bounds__(kYW) __attribute__((amdgpu_waves_per_eu(EXACT ? 4 : 1)))
+void k_stage_ymarch(YGrid m, int strips, int ch, int64_t units, int64_t nblocks, const double* __restrict__ Gs,
+                    const double* __restrict__ Gn, double* __restrict__ out, const double* __restrict__ u,
+                    const double* __restrict__ v, const double* __restrict__ sl) {
     __shared__ YLds<EXACT> L;
     const int64_t t = xcd_tile(blockIdx.x, nblocks);
     if (t >= units) return;
@@ -627,12 +628,14 @@ struct HGeom {
     double dx2, invdx, invdy, invdz, dtau, beta;
 };
 
-// One HCR-2 pseudo-time update of a cell (reinitialization.h:136-193, 3D) from
-// phi_n (P) and phi_0 (Q) at the cell and its neighbours in flag order
-// x-, x+, y-, y+, z-, z+ (reinitialization.h:46-89).  Crossing flags and r~
-// come from phi_0 with the arithmetic of computeInterfaceCrossings.
-// The part of a point update that depends on phi_0 alone: crossing flags,
-// the smoothed sign S and r~.  Both iterations of an invocation share it.
+// One HCR-2 pseudo-time update of a cell (reinitialization.h:136-193, 3D),
+// hcr2_point, from phi_n (P) and phi_0 (Q) at the cell and its neighbours in
+// flag order x-, x+, y-, y+, z-, z+ (reinitialization.h:46-89; entries 5 and 6
+// unused in 2D).  Crossing flags and r~ come from phi_0 with the arithmetic of
+// computeInterfaceCrossings.  The update splits into hcr2_q, the part that
+// depends on phi_0 alone (crossing flags, the smoothed sign S, r~), which both
+// iterations of an invocation share, and hcr2_p, the rest.  H is HGeom or
+// HPar (the members dx2, invdx, invdy, invdz, dtau, beta).
 struct HQ {
     double S, rt;
     int flag;
